@@ -1,0 +1,85 @@
+// bf_kernels.h — argument blocks and launchers of the HIP kernels.
+//
+// The kernels are templates over the velocity set (one translation unit per
+// set, bf_step.hip compiled with -DBF_VS=<id>), the storage type, the tick
+// parity and three feature switches; everything else is a runtime argument.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bf {
+
+enum Storage : int { ST_FP16 = 0, ST_FP32 = 1, ST_BF16 = 2 };
+
+// Per-object wall data, one entry per boundary id (LBM.cpp:613-629 holds the
+// same quantities; only the translational velocity, temperature and effective
+// density enter the update).
+struct alignas(16) ObjectEntry {
+    float ux, uy, uz, temperature;
+};
+
+struct StepArgs {
+    void *f;                 // lattice, Q slots of `stride` elements (storage type)
+    float *g;                // thermal lattice, QT slots of `stride` floats (or null)
+    int64_t stride;          // elements between slots
+    int X, Y, Z;
+    int periodic_x, periodic_y, periodic_z;
+    const uint8_t *ids;      // boundary id per cell (or null)
+    const ObjectEntry *objects;  // 256 entries
+    int halfway;             // 1: halfway bounce-back redirect, 0: full-way (solid cells skipped only)
+    int forcing;             // 0 none, 1 constant, 2 per-cell field
+    float fx, fy, fz;        // constant force
+    const float4 *force;     // per-cell force (xyz) when forcing == 2
+    float tau;
+    float ulim, uext;        // velocity clamp / equilibrium reset thresholds
+    int source_on;           // Guo source active (forcing || multiphase || thermal)
+    // Shan-Chen
+    float G;
+    const float *psi_prev;   // psi written on the previous tick
+    float *psi_cur;          // psi written on this tick
+    int write_psi;
+    // thermal
+    float tau_t, beta, t_ref, cs2_t;
+    // outputs
+    int update;              // write populations
+    float4 *vd;              // (u, rho) when non-null
+    float4 *ft;              // (F, T) when non-null
+    double *sums;            // 5 doubles accumulated when non-null
+};
+
+struct InitArgs {
+    void *f;
+    float *g;
+    int64_t stride;
+    int64_t n;
+    const float *rho;
+    const float *u;          // 3 per cell
+    const float *temperature;  // or null (T = 1)
+    const uint8_t *ids;      // or null
+    const ObjectEntry *objects;
+    const float *rho_eff;    // 256 entries
+    float *psi0, *psi1;      // or null
+    float cs2_t;
+};
+
+struct LaunchShape {
+    dim3 grid, block;
+    size_t lds_bytes;
+};
+
+// Block shape for a grid of X x Y x Z cells.  Multiphase kernels stage the
+// pseudopotential of the block tile plus a one-cell halo in LDS and use a
+// 64 x 4 tile; the others use up to 256 consecutive cells of an x-row.
+LaunchShape step_shape(int X, int Y, int Z, bool multiphase, bool three_d);
+
+// Launch one tick.  `partial` (5 doubles per block) receives FP64 block sums
+// of rho, raw momentum and T when non-null.  Returns hipSuccess or the launch error.
+template <int VS>
+hipError_t launch_step(const StepArgs &a, int storage, bool odd, bool walls, bool multiphase,
+                       bool thermal, const LaunchShape &shape, double *partial, hipStream_t stream);
+
+template <int VS>
+hipError_t launch_init(const InitArgs &a, int storage, bool thermal, hipStream_t stream);
+
+}  // namespace bf

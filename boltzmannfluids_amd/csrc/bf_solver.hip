@@ -1,0 +1,744 @@
+// bf_solver.hip — host runtime behind the C ABI of include/bf/bf.h.
+//
+// Owns the device lattice, the boundary / force / thermal / pseudopotential
+// fields and one HIP stream per handle, and sequences the fused tick kernel.
+// It takes the place of the reference's LBM host class
+// (BoltzmannFluids/Source/LBM/LBM.cpp): field initialisation with the same
+// detection rules (LBM.cpp:508-591), the object table (LBM.cpp:1015-1070),
+// the tick loop with parity (LBM.cpp:220-240,1557-1653) and field readback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/bf/bf.h"
+#include "bf_kernels.h"
+#include "bf_lattice.h"
+
+using namespace bf;
+
+struct bf_solver {
+    bf_config cfg{};
+    int vs = 0, q = 0, qt = 0;
+    int X = 0, Y = 0, Z = 0;
+    int64_t N = 0, stride = 0;
+    int storage = ST_FP32;
+    size_t elem = 4;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // device memory
+    void *f = nullptr;
+    float *g = nullptr;
+    uint8_t *ids = nullptr;
+    ObjectEntry *objects_d = nullptr;
+    float *rho_eff_d = nullptr;
+    float4 *force_d = nullptr;
+    float *psi[2] = {nullptr, nullptr};
+    float4 *vd = nullptr, *ft = nullptr;
+    double *partial = nullptr;
+    size_t partial_blocks = 0;
+    double *sums_d = nullptr;
+    int64_t device_bytes = 0;
+
+    // host state
+    ObjectEntry objects_h[256];
+    float rho_eff_h[256];
+    int object_count = 1;  // objects_cpu.size() in the reference (slot 0 = fluid)
+    float tau = 0.53f, G = -6.0f, beta = 0.5f, tau_t = 2.0f, t_ref = 1.0f, cs2_t = 1.0f / 3.0f;
+    float ulim = 0.25f, uext = 0.30f;
+    float fconst[3] = {0, 0, 0};
+    int forcing = 0, thermal = 0, has_ids = 0, initialized = 0;
+    int cur = 0;
+    int64_t ticks = 0;
+    LaunchShape shape{};
+    std::string err;
+};
+
+static std::string g_create_error;
+
+namespace {
+
+bf_status fail(bf_solver *s, bf_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (s) s->err = buf; else g_create_error = buf;
+    return st;
+}
+
+#define BF_HIP(s, call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((s), e_ == hipErrorOutOfMemory ? BF_ERR_OOM : BF_ERR_DEVICE,          \
+                        "[LBM Error] %s failed: %s", #call, hipGetErrorString(e_));            \
+    } while (0)
+
+template <class T>
+bf_status dalloc(bf_solver *s, T **p, size_t bytes) {
+    BF_HIP(s, hipMalloc((void **)p, bytes));
+    s->device_bytes += (int64_t)bytes;
+    return BF_OK;
+}
+
+void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+template <int VS>
+void table_of(float *out) {
+    using L = Lattice<VS>;
+    for (int i = 0; i < L::Q; ++i) {
+        out[4 * i + 0] = L::c[i][0];
+        out[4 * i + 1] = L::c[i][1];
+        out[4 * i + 2] = L::c[i][2];
+        out[4 * i + 3] = L::w[i];
+    }
+}
+
+float weight_of(int vs, int i) {
+    switch (vs) {
+    case D2Q9: return Lattice<D2Q9>::w[i];
+    case D3Q15: return Lattice<D3Q15>::w[i];
+    case D3Q19: return Lattice<D3Q19>::w[i];
+    case D3Q27: return Lattice<D3Q27>::w[i];
+    }
+    return 0.0f;
+}
+
+hipError_t dispatch_step(const bf_solver *s, const StepArgs &a, bool odd, double *partial) {
+    const bool walls = s->has_ids != 0, mp = s->cfg.multiphase != 0, th = s->thermal != 0;
+    switch (s->vs) {
+    case D2Q9: return launch_step<D2Q9>(a, s->storage, odd, walls, mp, th, s->shape, partial, s->stream);
+    case D3Q15: return launch_step<D3Q15>(a, s->storage, odd, walls, mp, th, s->shape, partial, s->stream);
+    case D3Q19: return launch_step<D3Q19>(a, s->storage, odd, walls, mp, th, s->shape, partial, s->stream);
+    case D3Q27: return launch_step<D3Q27>(a, s->storage, odd, walls, mp, th, s->shape, partial, s->stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t dispatch_init(const bf_solver *s, const InitArgs &a) {
+    const bool th = s->thermal != 0;
+    switch (s->vs) {
+    case D2Q9: return launch_init<D2Q9>(a, s->storage, th, s->stream);
+    case D3Q15: return launch_init<D3Q15>(a, s->storage, th, s->stream);
+    case D3Q19: return launch_init<D3Q19>(a, s->storage, th, s->stream);
+    case D3Q27: return launch_init<D3Q27>(a, s->storage, th, s->stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+// Deterministic second stage of the FP64 conservation sums: one block adds the
+// per-block partials in a fixed order.
+__global__ void __launch_bounds__(256) k_reduce_partials(const double *__restrict__ partial, size_t nblocks,
+                                                         double *__restrict__ out) {
+    __shared__ double red[256][5];
+    const int t = threadIdx.x;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (size_t b = t; b < nblocks; b += 256)
+        for (int k = 0; k < 5; ++k) acc[k] += partial[b * 5 + k];
+    for (int k = 0; k < 5; ++k) red[t][k] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 5; ++k) red[t][k] += red[t + w][k];
+        __syncthreads();
+    }
+    if (t < 5) out[t] = red[0][t];
+}
+
+StepArgs make_args(bf_solver *s, bool update) {
+    StepArgs a{};
+    a.f = s->f;
+    a.g = s->g;
+    a.stride = s->stride;
+    a.X = s->X; a.Y = s->Y; a.Z = s->Z;
+    a.periodic_x = s->cfg.periodic[0] != 0;
+    a.periodic_y = s->cfg.periodic[1] != 0;
+    a.periodic_z = s->cfg.periodic[2] != 0;
+    a.ids = s->ids;
+    a.objects = s->objects_d;
+    a.halfway = s->cfg.boundary_mode == BF_BB_HALFWAY;
+    a.forcing = s->forcing;
+    a.fx = s->fconst[0]; a.fy = s->fconst[1]; a.fz = s->fconst[2];
+    a.force = s->force_d;
+    a.tau = s->tau;
+    a.ulim = s->ulim; a.uext = s->uext;
+    a.source_on = (s->forcing != 0 || s->cfg.multiphase || s->thermal) ? 1 : 0;
+    a.G = s->G;
+    if (s->cfg.multiphase) {
+        a.psi_prev = s->psi[s->cur ^ 1];
+        a.psi_cur = s->psi[s->cur];
+    }
+    a.write_psi = 1;
+    a.tau_t = s->tau_t; a.beta = s->beta; a.t_ref = s->t_ref; a.cs2_t = s->cs2_t;
+    a.update = update ? 1 : 0;
+    return a;
+}
+
+bf_status upload_objects(bf_solver *s) {
+    if (!s->objects_d) return BF_OK;
+    BF_HIP(s, hipMemcpyAsync(s->objects_d, s->objects_h, sizeof(ObjectEntry) * 256, hipMemcpyHostToDevice, s->stream));
+    BF_HIP(s, hipMemcpyAsync(s->rho_eff_d, s->rho_eff_h, sizeof(float) * 256, hipMemcpyHostToDevice, s->stream));
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    return BF_OK;
+}
+
+float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    float v;
+    if (e == 0) v = std::ldexp((float)m, -24);
+    else if (e == 31) v = m ? NAN : INFINITY;
+    else v = std::ldexp((float)(m | 0x400u), (int)e - 25);
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    b |= sign;
+    std::memcpy(&v, &b, 4);
+    return v;
+}
+
+bf_status check_solver(const bf_solver *s) {
+    return s ? BF_OK : BF_ERR_INVALID;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int32_t bf_abi_version(void) { return BF_ABI_VERSION; }
+
+int32_t bf_velocity_count(int32_t vs) { return velocity_count(vs); }
+
+bf_status bf_velocity_table(int32_t set, int32_t thermal, float *out, int32_t capacity) {
+    if (!out) return BF_ERR_INVALID;
+    if (thermal) {
+        if (set == BF_D2Q5) {
+            if (capacity < 20) return BF_ERR_INVALID;
+            float t[36];
+            table_of<D2Q9>(t);
+            for (int k = 0; k < 5; ++k) {
+                out[4 * k] = t[4 * k]; out[4 * k + 1] = t[4 * k + 1]; out[4 * k + 2] = t[4 * k + 2];
+                out[4 * k + 3] = Lattice<D2Q9>::wt[k];
+            }
+            return BF_OK;
+        }
+        if (set == BF_D3Q7) {
+            if (capacity < 28) return BF_ERR_INVALID;
+            float t[76];
+            table_of<D3Q19>(t);
+            for (int k = 0; k < 7; ++k) {
+                out[4 * k] = t[4 * k]; out[4 * k + 1] = t[4 * k + 1]; out[4 * k + 2] = t[4 * k + 2];
+                out[4 * k + 3] = Lattice<D3Q19>::wt[k];
+            }
+            return BF_OK;
+        }
+        return BF_ERR_INVALID;
+    }
+    const int q = velocity_count(set);
+    if (!q || capacity < 4 * q) return BF_ERR_INVALID;
+    switch (set) {
+    case D2Q9: table_of<D2Q9>(out); break;
+    case D3Q15: table_of<D3Q15>(out); break;
+    case D3Q19: table_of<D3Q19>(out); break;
+    case D3Q27: table_of<D3Q27>(out); break;
+    }
+    return BF_OK;
+}
+
+const char *bf_last_error(const bf_solver *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+bf_status bf_create(const bf_config *cfg, bf_solver **out) {
+    if (!cfg || !out) return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: null argument");
+    *out = nullptr;
+    const int q = velocity_count(cfg->velocity_set);
+    if (!q) return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: velocity set %d is not supported", cfg->velocity_set);
+    const int X = cfg->resolution[0], Y = cfg->resolution[1], Z = cfg->resolution[2];
+    // LBM.cpp:429-432
+    if (X <= 0 || Y <= 0 || Z <= 0)
+        return fail(nullptr, BF_ERR_INVALID,
+                    "[LBM Error] bf_create: resolution (%d, %d, %d) is not valid, all components must be greater than zero", X, Y, Z);
+    // LBM.cpp:434-437
+    if (cfg->velocity_set == BF_D2Q9 && Z != 1)
+        return fail(nullptr, BF_ERR_INVALID,
+                    "[LBM Error] bf_create: velocity set dimension doesn't match the resolution dimension (2-D set needs z == 1)");
+    if (cfg->storage != BF_STORE_FP32 && cfg->storage != BF_STORE_FP16 && cfg->storage != BF_STORE_BF16)
+        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: storage %d is not supported", cfg->storage);
+    if (cfg->boundary_mode != BF_BB_HALFWAY && cfg->boundary_mode != BF_BB_FULLWAY_COMPAT)
+        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: boundary mode %d is not supported", cfg->boundary_mode);
+    if (!(cfg->relaxation_time > 0.5f))
+        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: relaxation time %g must exceed 0.5", (double)cfg->relaxation_time);
+    const int64_t N = (int64_t)X * Y * Z;
+    if (N >= (int64_t(1) << 31))
+        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: %lld cells exceed the 2^31 cell index range", (long long)N);
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, BF_ERR_DEVICE, "[LBM Error] bf_create: no HIP device available");
+    if (cfg->device < 0 || cfg->device >= ndev)
+        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: device %d out of range (%d devices)", cfg->device, ndev);
+
+    bf_solver *s = new (std::nothrow) bf_solver();
+    if (!s) return fail(nullptr, BF_ERR_OOM, "[LBM Error] bf_create: host allocation failed");
+    s->cfg = *cfg;
+    s->vs = cfg->velocity_set;
+    s->q = q;
+    s->qt = (cfg->velocity_set == BF_D2Q9) ? 5 : 7;
+    s->X = X; s->Y = Y; s->Z = Z;
+    s->N = N;
+    s->stride = (N + 63) / 64 * 64;
+    s->storage = cfg->storage;
+    s->elem = cfg->storage == BF_STORE_FP32 ? 4 : 2;
+    s->tau = cfg->relaxation_time;
+    for (int i = 0; i < 256; ++i) {
+        s->objects_h[i] = ObjectEntry{0.0f, 0.0f, 0.0f, 1.0f};  // referance_temperature (LBM.h:30)
+        s->rho_eff_h[i] = 1.0f;                                 // referance_boundry_density (LBM.h:31)
+    }
+    auto bail = [&](bf_status st) {
+        g_create_error = s->err;
+        bf_destroy(s);
+        return st;
+    };
+    if (hipSetDevice(cfg->device) != hipSuccess) { fail(s, BF_ERR_DEVICE, "[LBM Error] hipSetDevice failed"); return bail(BF_ERR_DEVICE); }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+        fail(s, BF_ERR_DEVICE, "[LBM Error] stream/event creation failed");
+        return bail(BF_ERR_DEVICE);
+    }
+    bf_status st;
+    if ((st = dalloc(s, &s->f, (size_t)q * s->stride * s->elem)) != BF_OK) return bail(st);
+    if ((st = dalloc(s, &s->vd, sizeof(float4) * N)) != BF_OK) return bail(st);
+    if ((st = dalloc(s, &s->ft, sizeof(float4) * N)) != BF_OK) return bail(st);
+    if (hipMemsetAsync(s->vd, 0, sizeof(float4) * N, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->ft, 0, sizeof(float4) * N, s->stream) != hipSuccess) {
+        fail(s, BF_ERR_DEVICE, "[LBM Error] clearing macroscopic fields failed");
+        return bail(BF_ERR_DEVICE);
+    }
+    if ((st = dalloc(s, &s->objects_d, sizeof(ObjectEntry) * 256)) != BF_OK) return bail(st);
+    if ((st = dalloc(s, &s->rho_eff_d, sizeof(float) * 256)) != BF_OK) return bail(st);
+    if ((st = dalloc(s, &s->sums_d, sizeof(double) * 8)) != BF_OK) return bail(st);
+    const bool d3 = cfg->velocity_set != BF_D2Q9;
+    s->shape = step_shape(X, Y, Z, cfg->multiphase != 0, d3);
+    s->partial_blocks = (size_t)s->shape.grid.x * s->shape.grid.y * s->shape.grid.z;
+    if ((st = dalloc(s, &s->partial, sizeof(double) * 5 * s->partial_blocks)) != BF_OK) return bail(st);
+    if (cfg->multiphase) {
+        if ((st = dalloc(s, &s->psi[0], sizeof(float) * N)) != BF_OK) return bail(st);
+        if ((st = dalloc(s, &s->psi[1], sizeof(float) * N)) != BF_OK) return bail(st);
+    }
+    if ((st = upload_objects(s)) != BF_OK) return bail(st);
+    *out = s;
+    return BF_OK;
+}
+
+void bf_destroy(bf_solver *s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    dfree(s->f); dfree(s->g); dfree(s->ids); dfree(s->objects_d); dfree(s->rho_eff_d);
+    dfree(s->force_d); dfree(s->psi[0]); dfree(s->psi[1]); dfree(s->vd); dfree(s->ft);
+    dfree(s->partial); dfree(s->sums_d);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+bf_status bf_set_object(bf_solver *s, uint32_t id, const float velocity[3], const float angular_velocity[3],
+                        const float center_of_mass[3], float temperature, float effective_density) {
+    if (!s) return BF_ERR_INVALID;
+    // LBM.cpp:1024-1032
+    if (id > 255)
+        return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_object: boundry_id(%u) is greater than maximum(255)", id);
+    if (id == 0)
+        return fail(s, BF_ERR_INVALID,
+                    "[LBM Error] bf_set_object: boundry_id(0) is defined to be fluid, it cannot be treated as an object");
+    (void)angular_velocity;  // stored by the reference but not used by its update (stream.comp:563-565)
+    (void)center_of_mass;
+    const float zero[3] = {0, 0, 0};
+    const float *v = velocity ? velocity : zero;
+    s->objects_h[id] = ObjectEntry{v[0], v[1], v[2], temperature};
+    s->rho_eff_h[id] = effective_density;
+    if ((int)id + 1 > s->object_count) s->object_count = (int)id + 1;
+    return upload_objects(s);
+}
+
+bf_status bf_clear_objects(bf_solver *s) {
+    if (!s) return BF_ERR_INVALID;
+    for (int i = 0; i < 256; ++i) {
+        s->objects_h[i] = ObjectEntry{0.0f, 0.0f, 0.0f, 1.0f};
+        s->rho_eff_h[i] = 1.0f;
+    }
+    s->object_count = 1;
+    return upload_objects(s);
+}
+
+bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const float *force,
+                         const float *temperature, const uint8_t *boundary_id) {
+    if (!s) return BF_ERR_INVALID;
+    if (!rho || !u) return fail(s, BF_ERR_INVALID, "[LBM Error] bf_init_fields: rho and u are required");
+    const int64_t N = s->N;
+    BF_HIP(s, hipSetDevice(s->cfg.device));
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+
+    // boundaries (LBM.cpp:593-691): ids grow the object table like objects_cpu.resize
+    dfree(s->ids); s->ids = nullptr;
+    s->has_ids = 0;
+    if (boundary_id) {
+        int maxid = 0;
+        for (int64_t c = 0; c < N; ++c) maxid = std::max(maxid, (int)boundary_id[c]);
+        if (maxid > 0) {
+            s->has_ids = 1;
+            if (maxid + 1 > s->object_count) s->object_count = maxid + 1;
+            bf_status st;
+            if ((st = dalloc(s, &s->ids, (size_t)N)) != BF_OK) return st;
+            BF_HIP(s, hipMemcpy(s->ids, boundary_id, (size_t)N, hipMemcpyHostToDevice));
+        }
+    }
+    // force detection (LBM.cpp:531-541,562)
+    dfree(s->force_d); s->force_d = nullptr;
+    s->forcing = 0;
+    if (force) {
+        bool constant = true;
+        for (int64_t c = 1; c < N && constant; ++c)
+            if (force[3 * c] != force[0] || force[3 * c + 1] != force[1] || force[3 * c + 2] != force[2]) constant = false;
+        if (constant) {
+            s->fconst[0] = force[0]; s->fconst[1] = force[1]; s->fconst[2] = force[2];
+            s->forcing = (force[0] != 0.0f || force[1] != 0.0f || force[2] != 0.0f) ? 1 : 0;
+        } else {
+            s->forcing = 2;
+            std::vector<float4> tmp((size_t)N);
+            for (int64_t c = 0; c < N; ++c) tmp[c] = make_float4(force[3 * c], force[3 * c + 1], force[3 * c + 2], 0.0f);
+            bf_status st;
+            if ((st = dalloc(s, &s->force_d, sizeof(float4) * N)) != BF_OK) return st;
+            BF_HIP(s, hipMemcpy(s->force_d, tmp.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+        }
+    } else {
+        s->fconst[0] = s->fconst[1] = s->fconst[2] = 0.0f;  // lambda default force (LBM.h:43)
+    }
+    // thermal detection (LBM.cpp:542-545)
+    s->thermal = 0;
+    if (temperature) {
+        for (int64_t c = 0; c < N && !s->thermal; ++c) {
+            if (temperature[c] != temperature[0]) s->thermal = 1;
+            if (s->has_ids && boundary_id[c] && s->objects_h[boundary_id[c]].temperature != temperature[0]) s->thermal = 1;
+        }
+    } else if (s->has_ids) {
+        for (int64_t c = 0; c < N && !s->thermal; ++c)
+            if (boundary_id[c] && s->objects_h[boundary_id[c]].temperature != 1.0f) s->thermal = 1;
+    }
+    dfree(s->g); s->g = nullptr;
+    if (s->thermal) {
+        bf_status st;
+        if ((st = dalloc(s, &s->g, sizeof(float) * (size_t)s->qt * s->stride)) != BF_OK) return st;
+    }
+    bf_status st = upload_objects(s);
+    if (st != BF_OK) return st;
+
+    // upload rho, u, T and run the equilibrium initialisation
+    float *rho_d = nullptr, *u_d = nullptr, *t_d = nullptr;
+    if ((st = dalloc(s, &rho_d, sizeof(float) * N)) != BF_OK) return st;
+    if ((st = dalloc(s, &u_d, sizeof(float) * 3 * N)) != BF_OK) { dfree(rho_d); return st; }
+    if (temperature && s->thermal) {
+        if ((st = dalloc(s, &t_d, sizeof(float) * N)) != BF_OK) { dfree(rho_d); dfree(u_d); return st; }
+    }
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s->stream);
+        dfree(rho_d); dfree(u_d); dfree(t_d);
+        s->device_bytes -= (int64_t)(sizeof(float) * N * 4 + (t_d ? sizeof(float) * N : 0));
+    };
+    hipError_t e = hipMemcpy(rho_d, rho, sizeof(float) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(u_d, u, sizeof(float) * 3 * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess && t_d) e = hipMemcpy(t_d, temperature, sizeof(float) * N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { cleanup(); return fail(s, BF_ERR_DEVICE, "[LBM Error] upload failed: %s", hipGetErrorString(e)); }
+
+    InitArgs ia{};
+    ia.f = s->f; ia.g = s->g; ia.stride = s->stride; ia.n = N;
+    ia.rho = rho_d; ia.u = u_d; ia.temperature = t_d; ia.ids = s->ids;
+    ia.objects = s->objects_d; ia.rho_eff = s->rho_eff_d;
+    ia.psi0 = s->psi[0]; ia.psi1 = s->psi[1];
+    ia.cs2_t = s->cs2_t;
+    e = dispatch_init(s, ia);
+    if (e != hipSuccess) { cleanup(); return fail(s, BF_ERR_DEVICE, "[LBM Error] init kernel failed: %s", hipGetErrorString(e)); }
+
+    // save the macroscopic fields once without updating (LBM.cpp:502)
+    s->cur = 0;
+    s->ticks = 0;
+    StepArgs a = make_args(s, false);
+    a.vd = s->vd; a.ft = s->ft;
+    e = dispatch_step(s, a, false, nullptr);
+    if (e != hipSuccess) { cleanup(); return fail(s, BF_ERR_DEVICE, "[LBM Error] init pass failed: %s", hipGetErrorString(e)); }
+    if (s->cfg.multiphase) s->cur ^= 1;
+    e = hipStreamSynchronize(s->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] init failed: %s", hipGetErrorString(e));
+    s->initialized = 1;
+    return BF_OK;
+}
+
+bf_status bf_set_param(bf_solver *s, int32_t key, const float *v, int32_t n) {
+    if (!s || !v || n < 1) return s ? fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: bad value") : BF_ERR_INVALID;
+    switch (key) {
+    case BF_PARAM_RELAXATION_TIME:
+        if (!(v[0] > 0.5f)) return fail(s, BF_ERR_INVALID, "[LBM Error] relaxation time must exceed 0.5");
+        s->tau = v[0]; return BF_OK;
+    case BF_PARAM_CONSTANT_FORCE:
+        if (n < 3) return fail(s, BF_ERR_INVALID, "[LBM Error] constant force needs 3 values");
+        s->fconst[0] = v[0]; s->fconst[1] = v[1]; s->fconst[2] = v[2];
+        if (s->forcing != 2) s->forcing = (v[0] != 0.0f || v[1] != 0.0f || v[2] != 0.0f) ? 1 : 0;
+        return BF_OK;
+    case BF_PARAM_INTERACTION_STRENGTH: s->G = v[0]; return BF_OK;
+    case BF_PARAM_THERMAL_EXPANSION: s->beta = v[0]; return BF_OK;
+    case BF_PARAM_THERMAL_RELAXATION_TIME:
+        if (!(v[0] > 0.5f)) return fail(s, BF_ERR_INVALID, "[LBM Error] thermal relaxation time must exceed 0.5");
+        s->tau_t = v[0]; return BF_OK;
+    case BF_PARAM_REFERENCE_TEMPERATURE: s->t_ref = v[0]; return BF_OK;
+    case BF_PARAM_THERMAL_CS2:
+        if (!(v[0] > 0.0f)) return fail(s, BF_ERR_INVALID, "[LBM Error] thermal c_s^2 must be positive");
+        s->cs2_t = v[0]; return BF_OK;
+    case BF_PARAM_VELOCITY_LIMIT: s->ulim = v[0]; return BF_OK;
+    case BF_PARAM_VELOCITY_LIMIT_EXTREME: s->uext = v[0]; return BF_OK;
+    }
+    return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: unknown key %d", key);
+}
+
+bf_status bf_get_param(const bf_solver *s, int32_t key, float *v, int32_t n) {
+    if (!s || !v || n < 1) return BF_ERR_INVALID;
+    switch (key) {
+    case BF_PARAM_RELAXATION_TIME: v[0] = s->tau; return BF_OK;
+    case BF_PARAM_CONSTANT_FORCE:
+        if (n < 3) return BF_ERR_INVALID;
+        v[0] = s->fconst[0]; v[1] = s->fconst[1]; v[2] = s->fconst[2]; return BF_OK;
+    case BF_PARAM_INTERACTION_STRENGTH: v[0] = s->G; return BF_OK;
+    case BF_PARAM_THERMAL_EXPANSION: v[0] = s->beta; return BF_OK;
+    case BF_PARAM_THERMAL_RELAXATION_TIME: v[0] = s->tau_t; return BF_OK;
+    case BF_PARAM_REFERENCE_TEMPERATURE: v[0] = s->t_ref; return BF_OK;
+    case BF_PARAM_THERMAL_CS2: v[0] = s->cs2_t; return BF_OK;
+    case BF_PARAM_VELOCITY_LIMIT: v[0] = s->ulim; return BF_OK;
+    case BF_PARAM_VELOCITY_LIMIT_EXTREME: v[0] = s->uext; return BF_OK;
+    }
+    return BF_ERR_INVALID;
+}
+
+static bf_status run_ticks(bf_solver *s, int64_t n, uint32_t flags) {
+    for (int64_t k = 0; k < n; ++k) {
+        const bool last = k == n - 1;
+        StepArgs a = make_args(s, true);
+        if (last && (flags & BF_STEP_SAVE_MACRO)) { a.vd = s->vd; a.ft = s->ft; }
+        double *partial = (last && (flags & BF_STEP_CONSERVATION)) ? s->partial : nullptr;
+        hipError_t e = dispatch_step(s, a, (s->ticks & 1) != 0, partial);
+        if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] tick kernel launch failed: %s", hipGetErrorString(e));
+        if (partial) {
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);
+            e = hipGetLastError();
+            if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] reduction launch failed: %s", hipGetErrorString(e));
+        }
+        if (s->cfg.multiphase) s->cur ^= 1;
+        s->ticks++;
+    }
+    return BF_OK;
+}
+
+bf_status bf_step(bf_solver *s, int64_t n, uint32_t flags) {
+    if (!s) return BF_ERR_INVALID;
+    if (!s->initialized) return fail(s, BF_ERR_STATE, "[LBM Error] bf_step called before bf_init_fields");
+    if (n < 0) return fail(s, BF_ERR_INVALID, "[LBM Error] bf_step: negative tick count");
+    if (hipSetDevice(s->cfg.device) != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] hipSetDevice failed");
+    return run_ticks(s, n, flags);
+}
+
+bf_status bf_step_timed(bf_solver *s, int64_t n, uint32_t flags, double *elapsed_ms) {
+    if (!s || !elapsed_ms) return BF_ERR_INVALID;
+    if (!s->initialized) return fail(s, BF_ERR_STATE, "[LBM Error] bf_step_timed called before bf_init_fields");
+    if (n < 0) return fail(s, BF_ERR_INVALID, "[LBM Error] bf_step_timed: negative tick count");
+    BF_HIP(s, hipSetDevice(s->cfg.device));
+    BF_HIP(s, hipEventRecord(s->ev0, s->stream));
+    bf_status st = run_ticks(s, n, flags);
+    if (st != BF_OK) return st;
+    BF_HIP(s, hipEventRecord(s->ev1, s->stream));
+    BF_HIP(s, hipEventSynchronize(s->ev1));
+    float ms = 0.0f;
+    BF_HIP(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *elapsed_ms = ms;
+    return BF_OK;
+}
+
+int64_t bf_ticks(const bf_solver *s) { return s ? s->ticks : -1; }
+
+bf_status bf_synchronize(bf_solver *s) {
+    if (!s) return BF_ERR_INVALID;
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    return BF_OK;
+}
+
+size_t bf_field_bytes(const bf_solver *s, int32_t field) {
+    if (!s) return 0;
+    switch (field) {
+    case BF_FIELD_VELOCITY_DENSITY: return sizeof(float) * 4 * (size_t)s->N;
+    case BF_FIELD_BOUNDARY: return (size_t)s->N;
+    case BF_FIELD_FORCE_TEMPERATURE: return sizeof(float) * 4 * (size_t)s->N;
+    case BF_FIELD_POPULATIONS_RAW: return sizeof(float) * (size_t)s->q * s->N;
+    case BF_FIELD_THERMAL_RAW: return s->thermal ? sizeof(float) * (size_t)s->qt * s->N : 0;
+    case BF_FIELD_PSI: return s->cfg.multiphase ? sizeof(float) * (size_t)s->N : 0;
+    }
+    return 0;
+}
+
+bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes) {
+    if (!s || !dst) return BF_ERR_INVALID;
+    const size_t need = bf_field_bytes(s, field);
+    if (need == 0) return fail(s, BF_ERR_INVALID, "[LBM Error] bf_read_field: field %d does not exist for this solver", field);
+    if (bytes != need) return fail(s, BF_ERR_INVALID, "[LBM Error] bf_read_field: %zu bytes given, %zu needed", bytes, need);
+    if (!s->initialized) return fail(s, BF_ERR_STATE, "[LBM Error] bf_read_field called before bf_init_fields");
+    BF_HIP(s, hipSetDevice(s->cfg.device));
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    const size_t N = (size_t)s->N;
+    switch (field) {
+    case BF_FIELD_VELOCITY_DENSITY:
+        BF_HIP(s, hipMemcpy(dst, s->vd, need, hipMemcpyDeviceToHost));
+        return BF_OK;
+    case BF_FIELD_FORCE_TEMPERATURE:
+        BF_HIP(s, hipMemcpy(dst, s->ft, need, hipMemcpyDeviceToHost));
+        return BF_OK;
+    case BF_FIELD_BOUNDARY:
+        if (s->ids) BF_HIP(s, hipMemcpy(dst, s->ids, N, hipMemcpyDeviceToHost));
+        else std::memset(dst, 0, N);
+        return BF_OK;
+    case BF_FIELD_POPULATIONS_RAW: {
+        float *out = static_cast<float *>(dst);
+        if (s->storage == ST_FP32) {
+            for (int i = 0; i < s->q; ++i)
+                BF_HIP(s, hipMemcpy(out + i * N, static_cast<float *>(s->f) + (size_t)i * s->stride, sizeof(float) * N,
+                                    hipMemcpyDeviceToHost));
+            return BF_OK;
+        }
+        std::vector<uint16_t> tmp(N);
+        for (int i = 0; i < s->q; ++i) {
+            BF_HIP(s, hipMemcpy(tmp.data(), static_cast<uint16_t *>(s->f) + (size_t)i * s->stride, 2 * N,
+                                hipMemcpyDeviceToHost));
+            const float w = weight_of(s->vs, i);
+            for (size_t c = 0; c < N; ++c) {
+                float v;
+                if (s->storage == ST_FP16) v = half_to_float(tmp[c]);
+                else { uint32_t b = (uint32_t)tmp[c] << 16; std::memcpy(&v, &b, 4); }
+                out[i * N + c] = v + w;
+            }
+        }
+        return BF_OK;
+    }
+    case BF_FIELD_THERMAL_RAW: {
+        float *out = static_cast<float *>(dst);
+        for (int k = 0; k < s->qt; ++k)
+            BF_HIP(s, hipMemcpy(out + k * N, s->g + (size_t)k * s->stride, sizeof(float) * N, hipMemcpyDeviceToHost));
+        return BF_OK;
+    }
+    case BF_FIELD_PSI:
+        BF_HIP(s, hipMemcpy(dst, s->psi[s->cur ^ 1], need, hipMemcpyDeviceToHost));
+        return BF_OK;
+    }
+    return BF_ERR_INVALID;
+}
+
+bf_status bf_conservation(bf_solver *s, double out[5]) {
+    if (!s || !out) return BF_ERR_INVALID;
+    if (!s->initialized) return fail(s, BF_ERR_STATE, "[LBM Error] bf_conservation called before bf_init_fields");
+    BF_HIP(s, hipSetDevice(s->cfg.device));
+    StepArgs a = make_args(s, false);
+    a.write_psi = 0;
+    hipError_t e = dispatch_step(s, a, (s->ticks & 1) != 0, s->partial);
+    if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] conservation kernel failed: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);
+    BF_HIP(s, hipGetLastError());
+    double tmp[5];
+    BF_HIP(s, hipMemcpyAsync(tmp, s->sums_d, sizeof tmp, hipMemcpyDeviceToHost, s->stream));
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    std::memcpy(out, tmp, sizeof tmp);
+    if (!s->thermal) out[4] = 0.0;
+    return BF_OK;
+}
+
+bf_status bf_last_step_sums(bf_solver *s, double out[5]) {
+    if (!s || !out) return BF_ERR_INVALID;
+    double tmp[5];
+    BF_HIP(s, hipMemcpyAsync(tmp, s->sums_d, sizeof tmp, hipMemcpyDeviceToHost, s->stream));
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    std::memcpy(out, tmp, sizeof tmp);
+    if (!s->thermal) out[4] = 0.0;
+    return BF_OK;
+}
+
+bf_status bf_get_info(const bf_solver *s, bf_info *o) {
+    if (!s || !o) return BF_ERR_INVALID;
+    o->resolution[0] = s->X; o->resolution[1] = s->Y; o->resolution[2] = s->Z;
+    o->q = s->q;
+    o->q_thermal = s->thermal ? s->qt : 0;
+    o->storage = s->storage;
+    o->forcing = s->forcing;
+    o->thermal = s->thermal;
+    o->multiphase = s->cfg.multiphase;
+    o->has_boundaries = s->has_ids;
+    o->cells = s->N;
+    o->stride = s->stride;
+    o->device_bytes = s->device_bytes;
+    return BF_OK;
+}
+
+// bits = 2^ceil(log2(ceil(log2(object_count)))), 0 when only the fluid slot exists (LBM.cpp:566-573)
+int32_t bf_bits_per_boundary(int32_t object_count) {
+    if (object_count <= 1) return 0;
+    return (int32_t)std::exp2(std::ceil(std::log2f(std::ceil(std::log2f((float)object_count)))));
+}
+
+bf_status bf_pack_boundaries(const uint8_t *ids, int64_t n, int32_t bits, uint32_t *words) {
+    if (!ids || !words || n < 0 || !(bits == 1 || bits == 2 || bits == 4 || bits == 8)) return BF_ERR_INVALID;
+    const int64_t nw = (n * bits + 31) / 32;
+    std::memset(words, 0, sizeof(uint32_t) * nw);
+    const uint32_t mask = (1u << bits) - 1u;
+    for (int64_t c = 0; c < n; ++c) {
+        if (ids[c] > mask) return BF_ERR_INVALID;
+        const int64_t b = c * bits;
+        words[b / 32] |= (uint32_t)ids[c] << (b % 32);
+    }
+    return BF_OK;
+}
+
+bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, uint8_t *ids) {
+    if (!ids || !words || n < 0 || !(bits == 1 || bits == 2 || bits == 4 || bits == 8)) return BF_ERR_INVALID;
+    const uint32_t mask = (1u << bits) - 1u;
+    for (int64_t c = 0; c < n; ++c) {
+        const int64_t b = c * bits;
+        ids[c] = (uint8_t)((words[b / 32] >> (b % 32)) & mask);
+    }
+    return BF_OK;
+}
+
+}  // extern "C"
+
+namespace bf {
+
+LaunchShape step_shape(int X, int Y, int Z, bool multiphase, bool three_d) {
+    (void)three_d;
+    LaunchShape sh{};
+    if (multiphase) {
+        sh.block = dim3(64, 4, 1);
+    } else {
+        const int bx = std::min(256, (X + 63) / 64 * 64);
+        const int by = std::max(1, 256 / bx);
+        sh.block = dim3(bx, by, 1);
+    }
+    sh.grid = dim3((X + sh.block.x - 1) / sh.block.x, (Y + sh.block.y - 1) / sh.block.y, Z);
+    sh.lds_bytes = 0;
+    return sh;
+}
+
+}  // namespace bf

@@ -1,0 +1,454 @@
+// bf_step.hip — fused in-place lattice Boltzmann tick for gfx950.
+//
+// One thread per cell, structure-of-arrays populations (slot i of cell c at
+// f[i * stride + c]), updated in place with the AA ("esoteric pull") pattern of
+// BoltzmannFluids/Source/GLSL/LBM/collide.comp:244-369:
+//   even tick: read slot i at x - c_i, write slot opp(i) at x + c_i;
+//   odd tick:  read slot opp(i) at x,   write slot i at x.
+// Each cell reads exactly the slots it writes, so the update is race free
+// without a second lattice.  The per-cell work restates collide.comp:676-793:
+// density floor, constant / per-cell force, Boussinesq buoyancy, Shan-Chen
+// pseudopotential force, half-force velocity shift with clamps, BGK with the
+// Guo source, thermal BGK, macroscopic save.  Halfway bounce-back with the
+// moving-wall term (stream.comp:487-496,576-580) and thermal anti-bounce-back
+// (stream_thermal.comp:520-524) are applied by redirecting the even-tick
+// accesses of directions whose neighbour is solid to the cell's own opposite
+// slot; odd ticks stay local.
+//
+// This file is compiled once per velocity set with -DBF_VS=<VelocitySetId>.
+// Floating-point contraction is disabled at build time so that the FP32 path
+// reproduces the CPU restatement's operation order bit for bit.
+#include "bf_kernels.h"
+#include "bf_lattice.h"
+
+#ifndef BF_VS
+#error "compile with -DBF_VS=<velocity set id>"
+#endif
+
+namespace bf {
+namespace {
+
+constexpr int kMpBX = 64;   // multiphase tile: 64 x 4 cells, one z-plane
+constexpr int kMpBY = 4;
+constexpr int kTileX = kMpBX + 2, kTileY = kMpBY + 2;
+
+__device__ inline uint16_t bf16_rne(float v) {
+    uint32_t b = __float_as_uint(v);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+
+// Storage policies: FP32 as is; FP16 and BF16 hold f - w.
+template <int ST> struct Store;
+template <> struct Store<ST_FP32> {
+    using T = float;
+    __device__ static float load(const T *p, int idx, float) { return p[idx]; }
+    __device__ static void store(T *p, int idx, float v, float) { p[idx] = v; }
+};
+template <> struct Store<ST_FP16> {
+    using T = _Float16;
+    __device__ static float load(const T *p, int idx, float w) { return (float)p[idx] + w; }
+    __device__ static void store(T *p, int idx, float v, float w) { p[idx] = (_Float16)(v - w); }
+};
+template <> struct Store<ST_BF16> {
+    using T = uint16_t;
+    __device__ static float load(const T *p, int idx, float w) {
+        return __uint_as_float((uint32_t)p[idx] << 16) + w;
+    }
+    __device__ static void store(T *p, int idx, float v, float w) { p[idx] = bf16_rne(v - w); }
+};
+
+// c_i . v summed over the non-zero components of c_i in x, y, z order.  Equal
+// to the full three-term dot product (the skipped terms are signed zeros).
+template <int VS>
+__device__ inline float dotc(int i, float vx, float vy, float vz) {
+    using L = Lattice<VS>;
+    float r = 0.0f;
+    bool first = true;
+    if (L::c[i][0] != 0) { r = L::c[i][0] > 0 ? vx : -vx; first = false; }
+    if (L::c[i][1] != 0) {
+        float t = L::c[i][1] > 0 ? vy : -vy;
+        r = first ? t : r + t;
+        first = false;
+    }
+    if (L::c[i][2] != 0) {
+        float t = L::c[i][2] > 0 ? vz : -vz;
+        r = first ? t : r + t;
+    }
+    return r;
+}
+
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH>
+__global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restrict__ partial) {
+    using L = Lattice<VS>;
+    using S = Store<ST>;
+    using T = typename S::T;
+    constexpr int Q = L::Q;
+    constexpr int QT = L::QT;
+    constexpr bool D3 = L::DIM == 3;
+
+    T *__restrict__ f = static_cast<T *>(a.f);
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = blockIdx.x * blockDim.x + tx;
+    const int y = blockIdx.y * blockDim.y + ty;
+    const int z = blockIdx.z;
+    const bool inside = x < X && y < Y;
+
+    // Neighbour coordinates: index 0 = -1, 1 = 0, 2 = +1 along each axis.
+    int xs[3] = {x - 1, x, x + 1};
+    bool vx[3] = {true, true, true};
+    if (xs[0] < 0) { if (a.periodic_x) xs[0] += X; else vx[0] = false; }
+    if (xs[2] >= X) { if (a.periodic_x) xs[2] -= X; else vx[2] = false; }
+    int ys[3] = {y - 1, y, y + 1};
+    bool vy[3] = {true, true, true};
+    if (ys[0] < 0) { if (a.periodic_y) ys[0] += Y; else vy[0] = false; }
+    if (ys[2] >= Y) { if (a.periodic_y) ys[2] -= Y; else vy[2] = false; }
+    int zs[3] = {z, z, z};
+    bool vz[3] = {true, true, true};
+    if (D3) {
+        zs[0] = z - 1; zs[2] = z + 1;
+        if (zs[0] < 0) { if (a.periodic_z) zs[0] += Z; else vz[0] = false; }
+        if (zs[2] >= Z) { if (a.periodic_z) zs[2] -= Z; else vz[2] = false; }
+    }
+    const int ry[3] = {ys[0] * X, ys[1] * X, ys[2] * X};
+    const int XY = X * Y;
+    const int pz[3] = {zs[0] * XY, zs[1] * XY, zs[2] * XY};
+    const int self = x + ry[1] + pz[1];
+
+    // Shan-Chen: stage psi of the previous tick for the block tile + halo.
+    __shared__ float tile[MP ? kTileX * kTileY * 3 : 1];
+    if constexpr (MP) {
+        const int tid = ty * kMpBX + tx;
+        const int nplanes = D3 ? 3 : 1;
+        const int x0 = blockIdx.x * kMpBX - 1, y0 = blockIdx.y * kMpBY - 1;
+        for (int k = tid; k < kTileX * kTileY * nplanes; k += kMpBX * kMpBY) {
+            int lx = k % kTileX, ly = (k / kTileX) % kTileY, lz = k / (kTileX * kTileY);
+            int gx = ((x0 + lx) % X + X) % X;
+            int gy = ((y0 + ly) % Y + Y) % Y;
+            int gz = D3 ? ((z - 1 + lz) % Z + Z) % Z : z;
+            tile[k] = a.psi_prev[gx + gy * X + gz * XY];
+        }
+        __syncthreads();
+    }
+
+    bool active = inside;
+    if (WALLS && active) active = a.ids[self] == 0;  // solid cells skip (collide.comp:696-700)
+
+    double s_rho = 0.0, s_mx = 0.0, s_my = 0.0, s_mz = 0.0, s_t = 0.0;
+
+    if (active) {
+        // Boundary ids of the Q-1 neighbours x + c_i (even ticks only).
+        uint8_t nb[Q];
+        const bool need_nb = WALLS && !ODD && (a.halfway || TH);
+        if (need_nb) {
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                const int dx = L::c[i][0] + 1, dy = L::c[i][1] + 1, dz = L::c[i][2] + 1;
+                nb[i] = (vx[dx] && vy[dy] && vz[dz]) ? a.ids[xs[dx] + ry[dy] + pz[dz]] : 0;
+            }
+        }
+
+        // ---- AA read (collide.comp:244-305) ----
+        float fi[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const float w = L::w[i];
+            const int o = opposite(i);
+            if (ODD || i == 0) {
+                fi[i] = S::load(f + (size_t)(ODD ? o : 0) * a.stride, self, L::w[ODD ? o : 0]);
+            } else {
+                const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
+                if (!(vx[dx] && vy[dy] && vz[dz])) {
+                    fi[i] = 0.01f;  // open boundary read (collide.comp:297-298)
+                } else if (WALLS && a.halfway && nb[o] != 0) {
+                    const ObjectEntry ob = a.objects[nb[o]];
+                    const float cu = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
+                    fi[i] = S::load(f + (size_t)o * a.stride, self, L::w[o]) + (6.0f * w) * cu;
+                } else {
+                    fi[i] = S::load(f + (size_t)i * a.stride, xs[dx] + ry[dy] + pz[dz], w);
+                }
+            }
+        }
+
+        // ---- moments (collide.comp:383-396, 704-705) ----
+        float rho = fi[0];
+#pragma unroll
+        for (int i = 1; i < Q; ++i) rho += fi[i];
+        float mx = 0.0f, my = 0.0f, mz = 0.0f;
+        {
+            bool fx0 = true, fy0 = true, fz0 = true;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? fi[i] : -fi[i]; mx = fx0 ? t : mx + t; fx0 = false; }
+                if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? fi[i] : -fi[i]; my = fy0 ? t : my + t; fy0 = false; }
+                if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? fi[i] : -fi[i]; mz = fz0 ? t : mz + t; fz0 = false; }
+            }
+        }
+        if (partial) { s_rho = rho; s_mx = mx; s_my = my; s_mz = mz; }
+        rho = fmaxf(rho, 0.01f);
+
+        // ---- force (collide.comp:707-720) ----
+        float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
+        if (a.forcing == 1) { Fx = a.fx; Fy = a.fy; Fz = a.fz; }
+        else if (a.forcing == 2) { const float4 fv = a.force[self]; Fx = fv.x; Fy = fv.y; Fz = fv.z; }
+
+        float gi[QT];
+        float temp = 0.0f;
+        if constexpr (TH) {
+#pragma unroll
+            for (int k = 0; k < QT; ++k) {
+                const int o = opposite(k);
+                if (ODD || k == 0) {
+                    gi[k] = a.g[(size_t)(ODD ? o : 0) * a.stride + self];
+                } else {
+                    const int dx = 1 - L::c[k][0], dy = 1 - L::c[k][1], dz = 1 - L::c[k][2];
+                    if (!(vx[dx] && vy[dy] && vz[dz])) {
+                        gi[k] = a.g[(size_t)o * a.stride + self];
+                    } else if (WALLS && nb[o] != 0) {
+                        gi[k] = -a.g[(size_t)o * a.stride + self] +
+                                (2.0f * L::wt[k]) * a.objects[nb[o]].temperature;
+                    } else {
+                        gi[k] = a.g[(size_t)k * a.stride + xs[dx] + ry[dy] + pz[dz]];
+                    }
+                }
+            }
+            temp = gi[0];
+#pragma unroll
+            for (int k = 1; k < QT; ++k) temp += gi[k];
+            if (partial) s_t = temp;
+            const float b = -a.beta * (temp - a.t_ref);
+            const float bx = b * Fx, by = b * Fy, bz = b * Fz;
+            Fx += bx; Fy += by; Fz += bz;
+        }
+
+        float psi_self = 0.0f;
+        if constexpr (MP) {
+            psi_self = 1.0f - expf(-rho);
+            float ax = 0.0f, ay = 0.0f, az = 0.0f;
+            bool ax0 = true, ay0 = true, az0 = true;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
+                const int li = (tx + dx) + (ty + dy) * kTileX + (D3 ? dz : 0) * kTileX * kTileY;
+                const float pn = (vx[dx] && vy[dy] && vz[dz]) ? tile[li] : psi_self;
+                const float wp = L::w[i] * pn;
+                if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? wp : -wp; ax = ax0 ? t : ax + t; ax0 = false; }
+                if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? wp : -wp; ay = ay0 ? t : ay + t; ay0 = false; }
+                if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? wp : -wp; az = az0 ? t : az + t; az0 = false; }
+            }
+            const float k = -psi_self * a.G;
+            Fx += ax * k; Fy += ay * k; Fz += az * k;
+        }
+
+        // ---- velocity with half-force shift and clamps (collide.comp:722-725) ----
+        const float r2 = 2.0f * rho;
+        float ux = mx / rho + Fx / r2;
+        float uy = my / rho + Fy / r2;
+        float uz = mz / rho + Fz / r2;
+        const float un = sqrtf(ux * ux + uy * uy + uz * uz);
+        const bool ext = un > a.uext;
+        if (un > a.ulim) {
+            const float sc = a.ulim / un;
+            ux *= sc; uy *= sc; uz *= sc;
+        }
+
+        // ---- BGK + Guo source, AA write (collide.comp:727-745, 307-369) ----
+        if (a.update) {
+            const float uu = ux * ux + uy * uy + uz * uz;
+            const float fu = Fx * ux + Fy * uy + Fz * uz;
+            const float pre = 1.0f - 1.0f / (2.0f * a.tau);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const float w = L::w[i];
+                const int o = opposite(i);
+                const float cu = (i == 0) ? 0.0f : dotc<BF_VS>(i, ux, uy, uz);
+                const float e = w * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+                float v;
+                if (ext) {
+                    v = e;
+                } else {
+                    v = fi[i] - (fi[i] - e) / a.tau;
+                    if (a.source_on) {
+                        const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
+                        v += pre * w * (3.0f * cf + 9.0f * cf * cu - 3.0f * fu);
+                    }
+                }
+                if (ODD || i == 0) {
+                    S::store(f + (size_t)i * a.stride, self, v, w);
+                } else {
+                    const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
+                    if (!(vx[dx] && vy[dy] && vz[dz])) {
+                        // open boundary: write dropped (collide.comp:358-359)
+                    } else if (WALLS && a.halfway && nb[i] != 0) {
+                        const ObjectEntry ob = a.objects[nb[i]];
+                        const float cw = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
+                        S::store(f + (size_t)i * a.stride, self, v - (6.0f * w) * cw, w);
+                    } else {
+                        S::store(f + (size_t)o * a.stride, xs[dx] + ry[dy] + pz[dz], v, w);
+                    }
+                }
+            }
+        }
+
+        if constexpr (MP) {
+            if (a.write_psi) a.psi_cur[self] = psi_self;  // collide.comp:763-769 (psi of rho)
+        }
+
+        // ---- thermal BGK (collide.comp:771-782) ----
+        if constexpr (TH) {
+            if (a.update) {
+                const float uu = ux * ux + uy * uy + uz * uz;
+                const float cs2 = a.cs2_t;
+#pragma unroll
+                for (int k = 0; k < QT; ++k) {
+                    const int o = opposite(k);
+                    const float cu = (k == 0) ? 0.0f : dotc<BF_VS>(k, ux, uy, uz);
+                    const float e = L::wt[k] * temp *
+                                    (1.0f + cu / cs2 + cu * cu / (2.0f * cs2 * cs2) - uu / (2.0f * cs2));
+                    const float v = gi[k] - (gi[k] - e) / a.tau_t;
+                    if (ODD || k == 0) {
+                        a.g[(size_t)k * a.stride + self] = v;
+                    } else {
+                        const int dx = 1 + L::c[k][0], dy = 1 + L::c[k][1], dz = 1 + L::c[k][2];
+                        if (!(vx[dx] && vy[dy] && vz[dz])) {
+                            a.g[(size_t)k * a.stride + self] = v;
+                        } else if (WALLS && nb[k] != 0) {
+                            a.g[(size_t)k * a.stride + self] =
+                                -v + (2.0f * L::wt[k]) * a.objects[nb[k]].temperature;
+                        } else {
+                            a.g[(size_t)o * a.stride + xs[dx] + ry[dy] + pz[dz]] = v;
+                        }
+                    }
+                }
+            }
+        }
+
+        // ---- macroscopic save (collide.comp:784-792) ----
+        if (a.vd) a.vd[self] = make_float4(ux, uy, uz, rho);
+        if (a.ft) a.ft[self] = make_float4(Fx, Fy, Fz, temp);
+    }
+
+    // ---- FP64 block partial sums (deterministic two-stage reduction) ----
+    if (partial) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            s_rho += __shfl_xor(s_rho, off);
+            s_mx += __shfl_xor(s_mx, off);
+            s_my += __shfl_xor(s_my, off);
+            s_mz += __shfl_xor(s_mz, off);
+            s_t += __shfl_xor(s_t, off);
+        }
+        __shared__ double red[4][5];
+        const int tid = ty * blockDim.x + tx;
+        const int wave = tid >> 6, lane = tid & 63;
+        if (lane == 0) {
+            red[wave][0] = s_rho; red[wave][1] = s_mx; red[wave][2] = s_my;
+            red[wave][3] = s_mz; red[wave][4] = s_t;
+        }
+        __syncthreads();
+        if (tid < 5) {
+            const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+            double acc = 0.0;
+            for (int wv = 0; wv < nw; ++wv) acc += red[wv][tid];
+            const size_t b = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+            partial[b * 5 + tid] = acc;
+        }
+    }
+}
+
+// Equilibrium initialisation (set_equilibrium_populations.comp:198-220; the
+// thermal lattice likewise, set_equilibrium_populations_thermal.comp:158-180,
+// indexed by population).  Solid cells start at rest (LBM.cpp:534-535); psi of
+// solid cells is psi(effective density) and never changes.
+template <int VS, int ST, bool TH>
+__global__ void __launch_bounds__(256) k_init(const InitArgs a) {
+    using L = Lattice<VS>;
+    using S = Store<ST>;
+    using T = typename S::T;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    T *f = static_cast<T *>(a.f);
+    const float r = a.rho[c];
+    float ux = a.u[3 * c], uy = a.u[3 * c + 1], uz = a.u[3 * c + 2];
+    const uint8_t id = a.ids ? a.ids[c] : 0;
+    if (id) { ux = 0.0f; uy = 0.0f; uz = 0.0f; }
+    const float uu = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const float cu = (float)L::c[i][0] * ux + (float)L::c[i][1] * uy + (float)L::c[i][2] * uz;
+        const float e = L::w[i] * r * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+        S::store(f + (size_t)i * a.stride, (int)c, e, L::w[i]);
+    }
+    if constexpr (TH) {
+        const float T0 = a.temperature ? a.temperature[c] : 1.0f;
+        const float cs2 = a.cs2_t;
+#pragma unroll
+        for (int k = 0; k < L::QT; ++k) {
+            const float cu = (float)L::c[k][0] * ux + (float)L::c[k][1] * uy + (float)L::c[k][2] * uz;
+            const float e = L::wt[k] * T0 * (1.0f + cu / cs2 + cu * cu / (2.0f * cs2 * cs2) - uu / (2.0f * cs2));
+            a.g[(size_t)k * a.stride + c] = e;
+        }
+    }
+    if (a.psi0) {
+        const float rr = id ? a.rho_eff[id] : r;
+        const float p = 1.0f - expf(-rr);
+        a.psi0[c] = p;
+        a.psi1[c] = p;
+    }
+}
+
+template <int ST, bool ODD>
+hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const LaunchShape &sh,
+                          double *partial, hipStream_t stream) {
+    constexpr int VS = BF_VS;
+#define BF_L(W, M, T)                                                                              \
+    do {                                                                                           \
+        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T>), sh.grid, sh.block, 0, stream, a, partial); \
+        return hipGetLastError();                                                                  \
+    } while (0)
+    if (walls) {
+        if (mp) { if (th) BF_L(true, true, true); else BF_L(true, true, false); }
+        else { if (th) BF_L(true, false, true); else BF_L(true, false, false); }
+    } else {
+        if (mp) { if (th) BF_L(false, true, true); else BF_L(false, true, false); }
+        else { if (th) BF_L(false, false, true); else BF_L(false, false, false); }
+    }
+#undef BF_L
+}
+
+template <int ST>
+hipError_t launch_init_st(const InitArgs &a, bool th, hipStream_t stream) {
+    const dim3 block(256), grid((unsigned)((a.n + 255) / 256));
+    if (th) hipLaunchKernelGGL((k_init<BF_VS, ST, true>), grid, block, 0, stream, a);
+    else hipLaunchKernelGGL((k_init<BF_VS, ST, false>), grid, block, 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+template <>
+hipError_t launch_step<BF_VS>(const StepArgs &a, int storage, bool odd, bool walls, bool mp, bool th,
+                              const LaunchShape &sh, double *partial, hipStream_t stream) {
+    switch (storage) {
+    case ST_FP32: return odd ? launch_step_st<ST_FP32, true>(a, walls, mp, th, sh, partial, stream)
+                             : launch_step_st<ST_FP32, false>(a, walls, mp, th, sh, partial, stream);
+    case ST_FP16: return odd ? launch_step_st<ST_FP16, true>(a, walls, mp, th, sh, partial, stream)
+                             : launch_step_st<ST_FP16, false>(a, walls, mp, th, sh, partial, stream);
+    case ST_BF16: return odd ? launch_step_st<ST_BF16, true>(a, walls, mp, th, sh, partial, stream)
+                             : launch_step_st<ST_BF16, false>(a, walls, mp, th, sh, partial, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <>
+hipError_t launch_init<BF_VS>(const InitArgs &a, int storage, bool th, hipStream_t stream) {
+    switch (storage) {
+    case ST_FP32: return launch_init_st<ST_FP32>(a, th, stream);
+    case ST_FP16: return launch_init_st<ST_FP16>(a, th, stream);
+    case ST_BF16: return launch_init_st<ST_BF16>(a, th, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace bf

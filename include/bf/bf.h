@@ -1,0 +1,252 @@
+/*
+ * bf.h — C ABI of the MI355X lattice Boltzmann solver (libbfamd.so).
+ *
+ * This is the drop-in boundary for the per-tick update of the reference
+ * solver class `LBM` (BoltzmannFluids/Source/LBM/LBM.h:26-332).  Every entry
+ * point takes plain pointers and sizes, returns a bf_status, never throws, and
+ * copies host buffers synchronously.  A handle owns all device memory and one
+ * HIP stream; it is thread-affine (one thread at a time per handle).
+ *
+ * Semantics of one tick (all velocity sets, all storage types):
+ *   - the lattice is stored in place, one array per direction
+ *     (structure of arrays, f[i * stride + cell], cell = z*X*Y + y*X + x), and
+ *     is updated with the esoteric "AA" pattern of
+ *     BoltzmannFluids/Source/GLSL/LBM/collide.comp:244-369: on even ticks a
+ *     cell reads slot i at x - c_i and writes slot opp(i) at x + c_i, on odd
+ *     ticks it reads slot opp(i) at x and writes slot i at x;
+ *   - BGK collision with Guo forcing, velocity clamp and density floor as in
+ *     collide.comp:676-793;
+ *   - FP16/BF16 storage keeps the shifted value f_i - w_i and does all
+ *     arithmetic in FP32.
+ */
+#ifndef BF_BF_H
+#define BF_BF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BF_ABI_VERSION 1
+
+typedef enum bf_status {
+    BF_OK = 0,
+    BF_ERR_INVALID = 1,      /* bad argument (reference: "[LBM Error]" + ASSERT) */
+    BF_ERR_STATE = 2,        /* call out of order, e.g. bf_step before bf_init_fields */
+    BF_ERR_UNSUPPORTED = 3,  /* feature combination not built */
+    BF_ERR_DEVICE = 4,       /* a HIP call failed; bf_last_error has the text */
+    BF_ERR_OOM = 5           /* device allocation failed */
+} bf_status;
+
+/* Velocity sets; values match enum VelocitySet in
+ * BoltzmannFluids/Source/LBMConstants/VelocitySet.h:18-23.  Direction order and
+ * weights follow VelocitySet.cpp:53-141 exactly. */
+typedef enum bf_velocity_set {
+    BF_D2Q9 = 2,
+    BF_D3Q15 = 3,
+    BF_D3Q19 = 4,
+    BF_D3Q27 = 5
+} bf_velocity_set;
+
+/* Thermal (advection-diffusion) sets; VelocitySet.h:13-16, VelocitySet.cpp:180-203. */
+typedef enum bf_thermal_set {
+    BF_D2Q5 = 0,
+    BF_D3Q7 = 1
+} bf_thermal_set;
+
+/* Population storage.  BF_FP32 and BF_FP16 keep the numeric values of enum
+ * FloatingPointAccuracy (FloatingPointAccuracy.h:6-9); the reference stores
+ * FP16 unshifted, this library stores f - w (shifted) for both 16-bit types. */
+typedef enum bf_storage {
+    BF_STORE_FP16 = 0,
+    BF_STORE_FP32 = 1,
+    BF_STORE_BF16 = 2
+} bf_storage;
+
+/* Wall treatment for solid (boundary id != 0) cells. */
+typedef enum bf_boundary_mode {
+    /* Halfway bounce-back with the moving-wall (Ladd) term of
+     * stream.comp:487-496,576-580, applied inside the in-place AA update. */
+    BF_BB_HALFWAY = 0,
+    /* Reference esoteric behaviour (collide.comp:696-700): solid cells are
+     * skipped and populations return to their sender two ticks later
+     * (full-way bounce-back, no wall velocity). */
+    BF_BB_FULLWAY_COMPAT = 1
+} bf_boundary_mode;
+
+typedef struct bf_config {
+    int32_t resolution[3];     /* X, Y, Z (Z == 1 for D2Q9)           LBM.h:50 */
+    float relaxation_time;     /* tau                                  LBM.h:51 */
+    int32_t periodic[3];       /* periodic_x/y/z                       LBM.h:52-54 */
+    int32_t velocity_set;      /* bf_velocity_set                      LBM.h:55 */
+    int32_t storage;           /* bf_storage                           LBM.h:56 */
+    int32_t multiphase;        /* Shan-Chen pseudopotential on/off     LBM.h:57 */
+    int32_t boundary_mode;     /* bf_boundary_mode */
+    int32_t device;            /* HIP device ordinal */
+} bf_config;
+
+/* Scalar / vector parameters for bf_set_param.  Defaults are the reference's. */
+typedef enum bf_param {
+    BF_PARAM_RELAXATION_TIME = 0,        /* 1 float; LBM::set_relaxation_time, LBM.cpp:285 */
+    BF_PARAM_CONSTANT_FORCE = 1,         /* 3 floats; LBM::set_constant_force, LBM.cpp:366 */
+    BF_PARAM_INTERACTION_STRENGTH = 2,   /* 1 float G (default -6); LBM.h:257, LBM.cpp:371 */
+    BF_PARAM_THERMAL_EXPANSION = 3,      /* 1 float beta (default 0.5); LBM.h:243 */
+    BF_PARAM_THERMAL_RELAXATION_TIME = 4,/* 1 float tau_T (default 2); LBM.cpp:461 */
+    BF_PARAM_REFERENCE_TEMPERATURE = 5,  /* 1 float T_ref (default 1); collide.comp:713 */
+    BF_PARAM_THERMAL_CS2 = 6,            /* 1 float c_s^2 of the thermal set (default 1/3); LBM.cpp:1611 */
+    BF_PARAM_VELOCITY_LIMIT = 7,         /* 1 float (default 0.25); LBM.h:144 */
+    BF_PARAM_VELOCITY_LIMIT_EXTREME = 8  /* 1 float (default 0.30); LBM.h:145 */
+} bf_param;
+
+/* Fields for bf_read_field. */
+typedef enum bf_field {
+    /* float4 per cell (u.x, u.y, u.z, rho) saved on the last BF_STEP_SAVE_MACRO
+     * tick; layout of velocity_density_texture (collide.comp:785, LBM.cpp:299). */
+    BF_FIELD_VELOCITY_DENSITY = 0,
+    /* uint8 per cell: boundary id (boundry_texture R8, LBM.cpp:300). */
+    BF_FIELD_BOUNDARY = 1,
+    /* float4 per cell (F.x, F.y, F.z, T); force_temperature_texture (collide.comp:790). */
+    BF_FIELD_FORCE_TEMPERATURE = 2,
+    /* Q*N floats: the in-place lattice exactly as stored (AA slot layout,
+     * population-major), converted to unshifted FP32. */
+    BF_FIELD_POPULATIONS_RAW = 3,
+    /* Qt*N floats: the in-place thermal lattice as stored (AA slot layout). */
+    BF_FIELD_THERMAL_RAW = 4,
+    /* N floats: pseudopotential psi written on the last tick (Shan-Chen). */
+    BF_FIELD_PSI = 5
+} bf_field;
+
+/* Flags for bf_step. */
+#define BF_STEP_SAVE_MACRO 1u     /* fill the macroscopic fields on the last tick (LBM.cpp:1626-1634) */
+#define BF_STEP_CONSERVATION 2u   /* FP64 sums of rho and rho*u over fluid cells on the last tick */
+
+typedef struct bf_solver bf_solver;
+
+/* --- lifetime ----------------------------------------------------------- */
+
+/* Library ABI version (BF_ABI_VERSION). */
+int32_t bf_abi_version(void);
+
+/* Number of populations Q of a velocity set, 0 if unknown
+ * (get_VelocitySet_vector_count, VelocitySet.cpp:21-35). */
+int32_t bf_velocity_count(int32_t velocity_set);
+
+/* Copy the direction table of a velocity set: 4 floats per direction
+ * (c.x, c.y, c.z, w) — the velocity_set_buffer UBO layout (LBM.cpp:869-875).
+ * `thermal` selects bf_thermal_set values instead.  out must hold 4*Q floats. */
+bf_status bf_velocity_table(int32_t set, int32_t thermal, float *out, int32_t capacity);
+
+/* Create a solver: validates the config and allocates the lattice.
+ * Replaces the allocation half of LBM::initialize_fields
+ * (LBM.cpp:415-462 and _generate_lattice_buffer, LBM.cpp:808-876). */
+bf_status bf_create(const bf_config *cfg, bf_solver **out);
+
+/* Release everything the handle owns.  Safe on NULL. */
+void bf_destroy(bf_solver *s);
+
+/* Last error text of this handle (or of the last failed bf_create when s is NULL). */
+const char *bf_last_error(const bf_solver *s);
+
+/* --- setup -------------------------------------------------------------- */
+
+/* Describe boundary object `id` (1..255): translational wall velocity,
+ * angular velocity, centre of mass, wall temperature and effective density.
+ * Replaces LBM::set_boundry_properties (LBM.h:60-80, LBM.cpp:1015-1065).
+ * May be called before or after bf_init_fields. */
+bf_status bf_set_object(bf_solver *s, uint32_t id, const float velocity[3],
+                        const float angular_velocity[3], const float center_of_mass[3],
+                        float temperature, float effective_density);
+
+/* Forget all objects (LBM::clear_boundry_properties, LBM.cpp:1067-1070). */
+bf_status bf_clear_objects(bf_solver *s);
+
+/* Initialise all fields from host arrays in cell order (cell = z*X*Y + y*X + x).
+ * Replaces the per-cell lambda of LBM::initialize_fields (LBM.cpp:415-506):
+ *   rho[N]; u[3N] (xyz per cell); force[3N] or NULL; temperature[N] or NULL;
+ *   boundary_id[N] or NULL (0 = fluid).
+ * As in the reference, the force is detected constant or varying
+ * (LBM.cpp:531-541,562), the flow is thermal when a temperature differs from
+ * the temperature of cell 0 (LBM.cpp:542-545), solid cells start at rest
+ * (LBM.cpp:534-535), populations start at equilibrium (set_equilibrium_populations.comp)
+ * and the macroscopic fields are saved once without updating the lattice
+ * (LBM.cpp:502).  Resets the tick counter to 0. */
+bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u,
+                         const float *force, const float *temperature,
+                         const uint8_t *boundary_id);
+
+/* Set a runtime parameter (see bf_param). */
+bf_status bf_set_param(bf_solver *s, int32_t key, const float *value, int32_t n);
+
+/* Read a runtime parameter. */
+bf_status bf_get_param(const bf_solver *s, int32_t key, float *value, int32_t n);
+
+/* --- iterate ------------------------------------------------------------ */
+
+/* Advance n ticks (n >= 0) with the fused in-place kernel.  Asynchronous on the
+ * handle's stream.  Replaces LBM::iterate_time / _iterate_time / _collide
+ * (LBM.cpp:220-240, 1484-1493, 1557-1653); wall-clock pacing stays with the
+ * caller.  flags: BF_STEP_SAVE_MACRO, BF_STEP_CONSERVATION (both act on the
+ * last of the n ticks). */
+bf_status bf_step(bf_solver *s, int64_t n, uint32_t flags);
+
+/* Same as bf_step but also times the n launches with HIP events recorded on the
+ * handle's stream; blocks until done and returns the elapsed milliseconds. */
+bf_status bf_step_timed(bf_solver *s, int64_t n, uint32_t flags, double *elapsed_ms);
+
+/* Ticks advanced since bf_init_fields (LBM::get_total_ticks_elapsed, LBM.cpp:242). */
+int64_t bf_ticks(const bf_solver *s);
+
+/* Wait for all work on the handle's stream. */
+bf_status bf_synchronize(bf_solver *s);
+
+/* --- readback ----------------------------------------------------------- */
+
+/* Copy a field to host memory (synchronous).  `bytes` must equal the field's
+ * size.  Replaces get_velocity_density_texture / get_boundry_texture /
+ * get_force_temperature_texture (LBM.h:97-99) for callers that read on the host. */
+bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes);
+
+/* Size in bytes of a field (0 if the field does not exist for this solver). */
+size_t bf_field_bytes(const bf_solver *s, int32_t field);
+
+/* FP64 conservation sums over fluid cells of the current state:
+ * out[0] = sum rho, out[1..3] = sum of sum_i f_i c_i (raw momentum, no
+ * half-force shift), out[4] = sum T (thermal flows, else 0).  Computed on the
+ * GPU from the populations the next tick will read.  No reference counterpart. */
+bf_status bf_conservation(bf_solver *s, double out[5]);
+
+/* Sums recorded by the last bf_step(..., BF_STEP_CONSERVATION), same layout. */
+bf_status bf_last_step_sums(bf_solver *s, double out[5]);
+
+/* Logical description of the lattice for interop and tests. */
+typedef struct bf_info {
+    int32_t resolution[3];
+    int32_t q;                 /* populations per cell */
+    int32_t q_thermal;         /* thermal populations per cell, 0 if not thermal */
+    int32_t storage;
+    int32_t forcing;           /* 0 none, 1 constant, 2 per-cell field */
+    int32_t thermal;
+    int32_t multiphase;
+    int32_t has_boundaries;
+    int64_t cells;
+    int64_t stride;            /* elements between two population slots */
+    int64_t device_bytes;      /* bytes of device memory the handle holds */
+} bf_info;
+
+bf_status bf_get_info(const bf_solver *s, bf_info *out);
+
+/* Pack/unpack boundary ids with `bits` (1, 2, 4 or 8) bits per cell in uint32
+ * words, the format of the reference's boundries buffer
+ * (LBM.cpp:566-573,631-659; collide.comp:401-418).  words must hold
+ * ceil(n*bits/32) entries.  Host-only helpers. */
+int32_t bf_bits_per_boundary(int32_t object_count);
+bf_status bf_pack_boundaries(const uint8_t *ids, int64_t n, int32_t bits, uint32_t *words);
+bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, uint8_t *ids);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BF_BF_H */

@@ -1,0 +1,728 @@
+/*
+ * bf_oracle.c — CPU restatement of the reference lattice Boltzmann tick.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP solver in
+ * boltzmannfluids_amd/csrc: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Built twice from this one source (see oracle/Makefile):
+ *   REAL=double -> liboracle_f64.so: the absolute FP64 oracle (populations
+ *                  stored in double whatever storage type is requested);
+ *   REAL=float  -> liboracle_f32.so: FP32 arithmetic in the same operation
+ *                  order as the GPU kernel, with FP16/BF16 shifted storage
+ *                  emulated bit for bit (round to nearest even of f - w).
+ *
+ * What it restates (reference paths relative to the repository root):
+ *   - direction tables: BoltzmannFluids/Source/LBMConstants/VelocitySet.cpp:53-141,180-203
+ *   - opposite direction (odd <-> odd+1): GLSL/LBM/collide.comp:219-222
+ *   - AA ("esoteric pull") read/write addressing: collide.comp:244-369
+ *     (even tick: read slot i at x-c_i, write slot opp(i) at x+c_i;
+ *      odd tick: read slot opp(i) at x, write slot i at x;
+ *      non-periodic out-of-range read -> 0.01, write dropped: :297-298,:358-359)
+ *   - the fused tick: collide.comp:676-793 (solid skip :696-700, density floor
+ *     :705, force :707-720, half-force velocity shift and clamps :722-725,
+ *     BGK + Guo source :727-745, Shan-Chen :534-584, thermal :771-782,
+ *     macroscopic save :784-792)
+ *   - equilibrium initialisation: set_equilibrium_populations.comp:179-220
+ *   - halfway bounce-back with moving wall (two-lattice reference):
+ *     stream.comp:487-496,576-584, here applied inside the AA update
+ *   - thermal anti-bounce-back: stream_thermal.comp:446-455,520-524
+ *   - textbook two-lattice pull streaming (stream.comp:498-584 without the
+ *     boundary branches) for the AA-equivalence test.
+ *   - field initialisation detection rules: LBM/LBM.cpp:508-591
+ *   - boundary bit packing: LBM.cpp:566-573,631-685 and collide.comp:401-418
+ *
+ * Parity pins (each checked in tests/): velocity-set closed forms, equilibrium
+ * and Guo-source moments, AA read/write sets equal per cell (race freedom),
+ * AA vs two-lattice equivalence, uniform-state fixed point, exact mass
+ * conservation, momentum gain N*F per tick, Taylor-Green decay, Poiseuille
+ * profile, bit-pack round trip.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL double
+#endif
+
+#define ORC_API __attribute__((visibility("default")))
+
+/* ---------------------------------------------------------------------- */
+/* Direction tables (VelocitySet.cpp:53-141, 180-203)                      */
+
+typedef struct { int c[3]; double w; } dir_t;
+
+static const dir_t D2Q9_T[9] = {
+    {{0, 0, 0}, 4.0 / 9}, {{1, 0, 0}, 1.0 / 9}, {{-1, 0, 0}, 1.0 / 9},
+    {{0, 1, 0}, 1.0 / 9}, {{0, -1, 0}, 1.0 / 9}, {{1, 1, 0}, 1.0 / 36},
+    {{-1, -1, 0}, 1.0 / 36}, {{1, -1, 0}, 1.0 / 36}, {{-1, 1, 0}, 1.0 / 36}};
+
+static const dir_t D3Q15_T[15] = {
+    {{0, 0, 0}, 2.0 / 9}, {{1, 0, 0}, 1.0 / 9}, {{-1, 0, 0}, 1.0 / 9},
+    {{0, 1, 0}, 1.0 / 9}, {{0, -1, 0}, 1.0 / 9}, {{0, 0, 1}, 1.0 / 9},
+    {{0, 0, -1}, 1.0 / 9}, {{1, 1, 1}, 1.0 / 72}, {{-1, -1, -1}, 1.0 / 72},
+    {{1, 1, -1}, 1.0 / 72}, {{-1, -1, 1}, 1.0 / 72}, {{1, -1, 1}, 1.0 / 72},
+    {{-1, 1, -1}, 1.0 / 72}, {{-1, 1, 1}, 1.0 / 72}, {{1, -1, -1}, 1.0 / 72}};
+
+static const dir_t D3Q19_T[19] = {
+    {{0, 0, 0}, 1.0 / 3}, {{1, 0, 0}, 1.0 / 18}, {{-1, 0, 0}, 1.0 / 18},
+    {{0, 1, 0}, 1.0 / 18}, {{0, -1, 0}, 1.0 / 18}, {{0, 0, 1}, 1.0 / 18},
+    {{0, 0, -1}, 1.0 / 18}, {{1, 1, 0}, 1.0 / 36}, {{-1, -1, 0}, 1.0 / 36},
+    {{1, -1, 0}, 1.0 / 36}, {{-1, 1, 0}, 1.0 / 36}, {{1, 0, 1}, 1.0 / 36},
+    {{-1, 0, -1}, 1.0 / 36}, {{1, 0, -1}, 1.0 / 36}, {{-1, 0, 1}, 1.0 / 36},
+    {{0, 1, 1}, 1.0 / 36}, {{0, -1, -1}, 1.0 / 36}, {{0, 1, -1}, 1.0 / 36},
+    {{0, -1, 1}, 1.0 / 36}};
+
+static const dir_t D3Q27_T[27] = {
+    {{0, 0, 0}, 8.0 / 27}, {{1, 0, 0}, 2.0 / 27}, {{-1, 0, 0}, 2.0 / 27},
+    {{0, 1, 0}, 2.0 / 27}, {{0, -1, 0}, 2.0 / 27}, {{0, 0, 1}, 2.0 / 27},
+    {{0, 0, -1}, 2.0 / 27}, {{1, 1, 0}, 1.0 / 54}, {{-1, -1, 0}, 1.0 / 54},
+    {{1, 0, 1}, 1.0 / 54}, {{-1, 0, -1}, 1.0 / 54}, {{0, 1, 1}, 1.0 / 54},
+    {{0, -1, -1}, 1.0 / 54}, {{1, -1, 0}, 1.0 / 54}, {{-1, 1, 0}, 1.0 / 54},
+    {{1, 0, -1}, 1.0 / 54}, {{-1, 0, 1}, 1.0 / 54}, {{0, 1, -1}, 1.0 / 54},
+    {{0, -1, 1}, 1.0 / 54}, {{1, 1, 1}, 1.0 / 216}, {{-1, -1, -1}, 1.0 / 216},
+    {{1, 1, -1}, 1.0 / 216}, {{-1, -1, 1}, 1.0 / 216}, {{1, -1, 1}, 1.0 / 216},
+    {{-1, 1, -1}, 1.0 / 216}, {{-1, 1, 1}, 1.0 / 216}, {{1, -1, -1}, 1.0 / 216}};
+
+static const dir_t D2Q5_T[5] = {
+    {{0, 0, 0}, 1.0 / 3}, {{1, 0, 0}, 1.0 / 6}, {{-1, 0, 0}, 1.0 / 6},
+    {{0, 1, 0}, 1.0 / 6}, {{0, -1, 0}, 1.0 / 6}};
+
+static const dir_t D3Q7_T[7] = {
+    {{0, 0, 0}, 1.0 / 4}, {{1, 0, 0}, 1.0 / 8}, {{-1, 0, 0}, 1.0 / 8},
+    {{0, 1, 0}, 1.0 / 8}, {{0, -1, 0}, 1.0 / 8}, {{0, 0, 1}, 1.0 / 8},
+    {{0, 0, -1}, 1.0 / 8}};
+
+/* enum values of VelocitySet.h:13-23 */
+enum { V_D2Q5 = 0, V_D3Q7 = 1, V_D2Q9 = 2, V_D3Q15 = 3, V_D3Q19 = 4, V_D3Q27 = 5 };
+
+static const dir_t *table_of(int set, int thermal, int *q) {
+    if (thermal) {
+        if (set == V_D2Q5) { *q = 5; return D2Q5_T; }
+        if (set == V_D3Q7) { *q = 7; return D3Q7_T; }
+        *q = 0; return NULL;
+    }
+    switch (set) {
+    case V_D2Q9: *q = 9; return D2Q9_T;
+    case V_D3Q15: *q = 15; return D3Q15_T;
+    case V_D3Q19: *q = 19; return D3Q19_T;
+    case V_D3Q27: *q = 27; return D3Q27_T;
+    }
+    *q = 0; return NULL;
+}
+
+/* collide.comp:219-222: opposite of i is i+1 for odd i, i-1 for even i, 0 for 0 */
+static int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+
+/* ---------------------------------------------------------------------- */
+/* Storage emulation for the FP32 build                                    */
+
+static float half_rne(float v) {
+    /* IEEE binary16 round-to-nearest-even of an FP32 value, returned as FP32. */
+    uint32_t b; memcpy(&b, &v, 4);
+    uint32_t sign = b & 0x80000000u;
+    uint32_t a = b & 0x7fffffffu;
+    float out;
+    if (a >= 0x7f800000u) { return v; } /* inf/nan pass through */
+    if (a >= 0x477ff000u) { /* >= 65520 rounds to inf */
+        uint32_t r = sign | 0x7f800000u; memcpy(&out, &r, 4); return out;
+    }
+    int e = (int)(a >> 23) - 127;
+    if (e >= -14) {
+        /* normal half: keep 10 mantissa bits */
+        uint32_t lsb = (a >> 13) & 1u;
+        uint32_t r = a + 0x0fffu + lsb;
+        r &= ~0x1fffu;
+        r |= sign;
+        memcpy(&out, &r, 4);
+        return out;
+    }
+    /* subnormal half: quantum 2^-24 */
+    {
+        double q = ldexp(1.0, -24);
+        double m = fabs((double)v) / q;
+        double fl = floor(m);
+        double d = m - fl;
+        double rr;
+        if (d > 0.5) rr = fl + 1.0;
+        else if (d < 0.5) rr = fl;
+        else rr = fmod(fl, 2.0) == 0.0 ? fl : fl + 1.0;
+        out = (float)(rr * q);
+        return sign ? -out : out;
+    }
+}
+
+static float bf16_rne(float v) {
+    uint32_t b; memcpy(&b, &v, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return v; /* nan */
+    uint32_t lsb = (b >> 16) & 1u;
+    b += 0x7fffu + lsb;
+    b &= 0xffff0000u;
+    float out; memcpy(&out, &b, 4);
+    return out;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Solver state                                                            */
+
+typedef struct orc_solver {
+    int X, Y, Z;
+    long N;
+    int q, qt;
+    const dir_t *T, *TT;
+    int vset, storage, multiphase, bb_mode;
+    int per[3];
+    /* parameters (reference defaults: LBM.h:144-145,243,257; LBM.cpp:461; collide.comp:713) */
+    REAL tau, G, beta, tau_t, t_ref, cs2_t, ulim, uext;
+    REAL fconst[3];
+    int forcing;   /* 0 none, 1 constant, 2 field */
+    int thermal;
+    int has_ids;
+    /* objects: id -> u_w[3], T_w, rho_eff */
+    REAL obj[256][5];
+    /* fields */
+    REAL *f;       /* q*N, AA slot layout */
+    REAL *g;       /* qt*N */
+    REAL *force;   /* 3N */
+    uint8_t *ids;  /* N */
+    REAL *psi[2];  /* N each; psi[cur^1] = previous tick */
+    int cur;
+    float *vd, *ft; /* 4N each, saved macro */
+    long t;
+    double sums[5];
+} orc_solver;
+
+static long cid(const orc_solver *s, int x, int y, int z) {
+    return ((long)z * s->Y + y) * s->X + x;
+}
+
+/* Wrap a coordinate on a periodic axis; returns 0 if out of range on an open axis. */
+static int wrap(const orc_solver *s, int axis, int *v) {
+    int n = axis == 0 ? s->X : (axis == 1 ? s->Y : s->Z);
+    if (s->per[axis]) {
+        /* positive_modulo, collide.comp:232-242 */
+        *v = ((*v % n) + n) % n;
+        return 1;
+    }
+    return (*v >= 0 && *v < n);
+}
+
+/* Neighbour x + sgn*c; returns 0 if outside the domain on an open axis. */
+static int nbr(const orc_solver *s, int x, int y, int z, const int *c, int sgn, long *out) {
+    int p[3] = {x + sgn * c[0], y + sgn * c[1], z + sgn * c[2]};
+    for (int a = 0; a < 3; ++a)
+        if (!wrap(s, a, &p[a])) return 0;
+    *out = cid(s, p[0], p[1], p[2]);
+    return 1;
+}
+
+/* Storage round trip of one population (shifted 16-bit in the FP32 build). */
+static REAL store_val(const orc_solver *s, int i, REAL v) {
+#ifdef ORACLE_F32
+    float w = (float)s->T[i].w;
+    if (s->storage == 0) return (REAL)(half_rne((float)v - w));
+    if (s->storage == 2) return (REAL)(bf16_rne((float)v - w));
+    return v;
+#else
+    (void)s; (void)i;
+    return v;
+#endif
+}
+static REAL load_val(const orc_solver *s, int i, REAL raw) {
+#ifdef ORACLE_F32
+    float w = (float)s->T[i].w;
+    if (s->storage == 0 || s->storage == 2) return (REAL)((float)raw + w);
+    return raw;
+#else
+    (void)s; (void)i;
+    return raw;
+#endif
+}
+
+#define F_AT(s, i, c) ((s)->f[(long)(i) * (s)->N + (c)])
+#define G_AT(s, k, c) ((s)->g[(long)(k) * (s)->N + (c)])
+
+static int is_solid(const orc_solver *s, long c) { return s->has_ids && s->ids[c] != 0; }
+
+/* Equilibrium, second order, c_s^2 = 1/3 (collide.comp:589-606 with exact 3, 9/2, 3/2
+ * as in set_equilibrium_populations.comp:179-196). */
+static REAL feq(REAL w, REAL rho, REAL cu, REAL uu) {
+    return w * rho * ((REAL)1 + (REAL)3 * cu + (REAL)4.5 * cu * cu - (REAL)1.5 * uu);
+}
+
+/* Thermal equilibrium with a configurable c_s^2 (collide.comp:634-651). */
+static REAL geq(REAL w, REAL T, REAL cu, REAL uu, REAL cs2) {
+    return w * T * ((REAL)1 + cu / cs2 + cu * cu / ((REAL)2 * cs2 * cs2) - uu / ((REAL)2 * cs2));
+}
+
+/* AA read of population i of the cell at (x,y,z) (collide.comp:244-305 + halfway BB). */
+static REAL read_pop(const orc_solver *s, int x, int y, int z, int i, int odd) {
+    long self = cid(s, x, y, z);
+    const int *c = s->T[i].c;
+    if (odd) return load_val(s, opp(i), F_AT(s, opp(i), self));
+    long src;
+    if (!nbr(s, x, y, z, c, -1, &src)) return (REAL)0.01;
+    if (s->bb_mode == 0 && is_solid(s, src)) {
+        const REAL *o = s->obj[s->ids[src]];
+        REAL cu = (REAL)c[0] * o[0] + (REAL)c[1] * o[1] + (REAL)c[2] * o[2];
+        /* f_i(x) = f*_opp(i)(x) - 2 w_opp rho_w (c_opp . u_w) * 3, rho_w = 1 (stream.comp:487-496,577-580) */
+        return load_val(s, opp(i), F_AT(s, opp(i), self)) + (REAL)6 * (REAL)s->T[i].w * cu;
+    }
+    return load_val(s, i, F_AT(s, i, src));
+}
+
+/* AA write of post-collision population i of the cell at (x,y,z) (collide.comp:307-369). */
+static void write_pop(orc_solver *s, int x, int y, int z, int i, int odd, REAL v) {
+    long self = cid(s, x, y, z);
+    const int *c = s->T[i].c;
+    if (odd) { F_AT(s, i, self) = store_val(s, i, v); return; }
+    long dst;
+    if (!nbr(s, x, y, z, c, +1, &dst)) return;
+    if (s->bb_mode == 0 && is_solid(s, dst)) {
+        const REAL *o = s->obj[s->ids[dst]];
+        REAL cu = (REAL)c[0] * o[0] + (REAL)c[1] * o[1] + (REAL)c[2] * o[2];
+        F_AT(s, i, self) = store_val(s, i, v - (REAL)6 * (REAL)s->T[i].w * cu);
+        return;
+    }
+    F_AT(s, opp(i), dst) = store_val(s, opp(i), v);
+}
+
+/* Thermal AA read/write with anti-bounce-back at solid neighbours
+ * (stream_thermal.comp:446-455,520-524) and zero-flux bounce-back at open edges. */
+static REAL read_th(const orc_solver *s, int x, int y, int z, int k, int odd) {
+    long self = cid(s, x, y, z);
+    const int *c = s->TT[k].c;
+    if (odd) return G_AT(s, opp(k), self);
+    long src;
+    if (!nbr(s, x, y, z, c, -1, &src)) return G_AT(s, opp(k), self);
+    if (is_solid(s, src))
+        return -G_AT(s, opp(k), self) + (REAL)2 * (REAL)s->TT[k].w * s->obj[s->ids[src]][3];
+    return G_AT(s, k, src);
+}
+
+static void write_th(orc_solver *s, int x, int y, int z, int k, int odd, REAL v) {
+    long self = cid(s, x, y, z);
+    const int *c = s->TT[k].c;
+    if (odd) { G_AT(s, k, self) = v; return; }
+    long dst;
+    if (!nbr(s, x, y, z, c, +1, &dst)) { G_AT(s, k, self) = v; return; }
+    if (is_solid(s, dst)) {
+        G_AT(s, k, self) = -v + (REAL)2 * (REAL)s->TT[k].w * s->obj[s->ids[dst]][3];
+        return;
+    }
+    G_AT(s, opp(k), dst) = v;
+}
+
+/* psi = rho0 (1 - exp(-rho/rho0)), rho0 = 1 (collide.comp:512-516) */
+#ifdef ORACLE_F32
+static REAL psi_of(REAL rho) { return 1.0f - expf(-rho); }
+static REAL vlen(REAL a, REAL b, REAL c) { return sqrtf(a * a + b * b + c * c); }
+#else
+static REAL psi_of(REAL rho) { return (REAL)1 - (REAL)exp(-(double)rho); }
+static REAL vlen(REAL a, REAL b, REAL c) { return (REAL)sqrt((double)(a * a + b * b + c * c)); }
+#endif
+
+/* One tick (collide.comp:676-793). */
+static void tick(orc_solver *s, int update, int save, int want_sums) {
+    int odd = (int)(s->t & 1);
+    const REAL *psi_prev = s->multiphase ? s->psi[s->cur ^ 1] : NULL;
+    REAL *psi_cur = s->multiphase ? s->psi[s->cur] : NULL;
+    double sm[5] = {0, 0, 0, 0, 0};
+    REAL f[27], g[7];
+    for (int z = 0; z < s->Z; ++z)
+    for (int y = 0; y < s->Y; ++y)
+    for (int x = 0; x < s->X; ++x) {
+        long cell = cid(s, x, y, z);
+        if (is_solid(s, cell)) continue;                 /* collide.comp:696-700 */
+        for (int i = 0; i < s->q; ++i) f[i] = read_pop(s, x, y, z, i, odd);
+        REAL rho = 0, m[3] = {0, 0, 0};
+        for (int i = 0; i < s->q; ++i) rho += f[i];      /* collide.comp:383-388 */
+        for (int i = 0; i < s->q; ++i) {                 /* collide.comp:390-396 */
+            m[0] += f[i] * (REAL)s->T[i].c[0];
+            m[1] += f[i] * (REAL)s->T[i].c[1];
+            m[2] += f[i] * (REAL)s->T[i].c[2];
+        }
+        if (want_sums) { sm[0] += (double)rho; sm[1] += (double)m[0]; sm[2] += (double)m[1]; sm[3] += (double)m[2]; }
+        if (rho < (REAL)0.01) rho = (REAL)0.01;          /* collide.comp:705 */
+        REAL F[3] = {0, 0, 0};
+        if (s->forcing == 1) { F[0] = s->fconst[0]; F[1] = s->fconst[1]; F[2] = s->fconst[2]; }
+        else if (s->forcing == 2) { F[0] = s->force[3 * cell]; F[1] = s->force[3 * cell + 1]; F[2] = s->force[3 * cell + 2]; }
+        REAL T = 0;
+        if (s->thermal) {                                 /* collide.comp:709-716 */
+            for (int k = 0; k < s->qt; ++k) g[k] = read_th(s, x, y, z, k, odd);
+            for (int k = 0; k < s->qt; ++k) T += g[k];
+            if (want_sums) sm[4] += (double)T;
+            REAL b = -s->beta * (T - s->t_ref);
+            REAL Fb[3] = {b * F[0], b * F[1], b * F[2]};
+            F[0] += Fb[0]; F[1] += Fb[1]; F[2] += Fb[2];
+        }
+        if (s->multiphase) {                             /* collide.comp:534-584 */
+            REAL ps = psi_of(rho);
+            REAL acc[3] = {0, 0, 0};
+            for (int i = 1; i < s->q; ++i) {
+                long n;
+                REAL pn = nbr(s, x, y, z, s->T[i].c, +1, &n) ? psi_prev[n] : ps;
+                REAL wp = (REAL)s->T[i].w * pn;
+                acc[0] += wp * (REAL)s->T[i].c[0];
+                acc[1] += wp * (REAL)s->T[i].c[1];
+                acc[2] += wp * (REAL)s->T[i].c[2];
+            }
+            REAL k = -ps * s->G;
+            F[0] += acc[0] * k; F[1] += acc[1] * k; F[2] += acc[2] * k;
+        }
+        REAL u[3];                                        /* collide.comp:722-725 */
+        for (int a = 0; a < 3; ++a) u[a] = m[a] / rho + F[a] / ((REAL)2 * rho);
+        REAL un = vlen(u[0], u[1], u[2]);
+        int ext = un > s->uext;
+        if (un > s->ulim) { REAL sc = s->ulim / un; u[0] *= sc; u[1] *= sc; u[2] *= sc; }
+        if (update) {                                     /* collide.comp:727-745 */
+            REAL uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            REAL fu = F[0] * u[0] + F[1] * u[1] + F[2] * u[2];
+            REAL pre = (REAL)1 - (REAL)1 / ((REAL)2 * s->tau);
+            int src_on = s->forcing || s->multiphase || s->thermal;
+            for (int i = 0; i < s->q; ++i) {
+                const int *c = s->T[i].c;
+                REAL w = (REAL)s->T[i].w;
+                REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
+                REAL e = feq(w, rho, cu, uu);
+                REAL v;
+                if (ext) v = e;
+                else {
+                    v = f[i] - (f[i] - e) / s->tau;
+                    if (src_on) {
+                        REAL cf = (REAL)c[0] * F[0] + (REAL)c[1] * F[1] + (REAL)c[2] * F[2];
+                        v += pre * w * ((REAL)3 * cf + (REAL)9 * cf * cu - (REAL)3 * fu);
+                    }
+                }
+                write_pop(s, x, y, z, i, odd, v);
+            }
+        }
+        if (s->multiphase) psi_cur[cell] = psi_of(rho);  /* collide.comp:763-769 (psi stored) */
+        if (s->thermal) {                                 /* collide.comp:771-782 */
+            REAL uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            for (int k = 0; k < s->qt; ++k) {
+                const int *c = s->TT[k].c;
+                REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
+                REAL e = geq((REAL)s->TT[k].w, T, cu, uu, s->cs2_t);
+                REAL v = g[k] - (g[k] - e) / s->tau_t;
+                if (update) write_th(s, x, y, z, k, odd, v);
+            }
+        }
+        if (save) {                                       /* collide.comp:784-792 */
+            s->vd[4 * cell + 0] = (float)u[0]; s->vd[4 * cell + 1] = (float)u[1];
+            s->vd[4 * cell + 2] = (float)u[2]; s->vd[4 * cell + 3] = (float)rho;
+            s->ft[4 * cell + 0] = (float)F[0]; s->ft[4 * cell + 1] = (float)F[1];
+            s->ft[4 * cell + 2] = (float)F[2]; s->ft[4 * cell + 3] = (float)T;
+        }
+    }
+    if (want_sums) memcpy(s->sums, sm, sizeof sm);
+    if (s->multiphase) s->cur ^= 1;
+}
+
+/* ---------------------------------------------------------------------- */
+/* API                                                                     */
+
+ORC_API int orc_velocity_table(int set, int thermal, double *out) {
+    int q; const dir_t *t = table_of(set, thermal, &q);
+    if (!t) return 0;
+    for (int i = 0; i < q; ++i) {
+        out[4 * i] = t[i].c[0]; out[4 * i + 1] = t[i].c[1]; out[4 * i + 2] = t[i].c[2];
+        out[4 * i + 3] = t[i].w;
+    }
+    return q;
+}
+
+ORC_API int orc_opposite(int i) { return opp(i); }
+
+ORC_API orc_solver *orc_create(int X, int Y, int Z, int vset, int storage, int multiphase,
+                               int bb_mode, int px, int py, int pz, double tau) {
+    int q; const dir_t *T = table_of(vset, 0, &q);
+    if (!T || X <= 0 || Y <= 0 || Z <= 0) return NULL;
+    if (vset == V_D2Q9 && Z != 1) return NULL;
+    orc_solver *s = (orc_solver *)calloc(1, sizeof *s);
+    s->X = X; s->Y = Y; s->Z = Z; s->N = (long)X * Y * Z;
+    s->q = q; s->T = T; s->vset = vset; s->storage = storage;
+    s->multiphase = multiphase; s->bb_mode = bb_mode;
+    s->per[0] = px; s->per[1] = py; s->per[2] = pz;
+    s->TT = table_of(vset == V_D2Q9 ? V_D2Q5 : V_D3Q7, 1, &s->qt);
+    s->tau = (REAL)tau; s->G = (REAL)-6.0; s->beta = (REAL)0.5; s->tau_t = (REAL)2;
+    s->t_ref = (REAL)1; s->cs2_t = (REAL)(1.0 / 3.0); s->ulim = (REAL)0.25; s->uext = (REAL)0.30;
+    for (int i = 0; i < 256; ++i) { s->obj[i][3] = 1; s->obj[i][4] = 1; }
+    s->f = (REAL *)calloc((size_t)q * s->N, sizeof(REAL));
+    s->vd = (float *)calloc((size_t)4 * s->N, sizeof(float));
+    s->ft = (float *)calloc((size_t)4 * s->N, sizeof(float));
+    return s;
+}
+
+ORC_API void orc_destroy(orc_solver *s) {
+    if (!s) return;
+    free(s->f); free(s->g); free(s->force); free(s->ids); free(s->psi[0]); free(s->psi[1]);
+    free(s->vd); free(s->ft); free(s);
+}
+
+/* keys follow bf_param in include/bf/bf.h */
+ORC_API int orc_set_param(orc_solver *s, int key, const double *v, int n) {
+    switch (key) {
+    case 0: s->tau = (REAL)v[0]; return 0;
+    case 1:
+        /* a constant force set after initialisation takes effect unless the
+         * force is a per-cell field (LBM::set_constant_force, LBM.cpp:366-369) */
+        if (n < 3) return 1;
+        s->fconst[0] = (REAL)v[0]; s->fconst[1] = (REAL)v[1]; s->fconst[2] = (REAL)v[2];
+        if (s->forcing != 2) s->forcing = (v[0] != 0 || v[1] != 0 || v[2] != 0) ? 1 : 0;
+        return 0;
+    case 2: s->G = (REAL)v[0]; return 0;
+    case 3: s->beta = (REAL)v[0]; return 0;
+    case 4: s->tau_t = (REAL)v[0]; return 0;
+    case 5: s->t_ref = (REAL)v[0]; return 0;
+    case 6: s->cs2_t = (REAL)v[0]; return 0;
+    case 7: s->ulim = (REAL)v[0]; return 0;
+    case 8: s->uext = (REAL)v[0]; return 0;
+    }
+    return 1;
+}
+
+/* Object table (LBM::set_boundry_properties, LBM.cpp:1015-1037); only the
+ * translational velocity, temperature and effective density enter the update. */
+ORC_API int orc_set_object(orc_solver *s, int id, const double *u, double T, double rho_eff) {
+    if (id <= 0 || id > 255) return 1;
+    s->obj[id][0] = (REAL)u[0]; s->obj[id][1] = (REAL)u[1]; s->obj[id][2] = (REAL)u[2];
+    s->obj[id][3] = (REAL)T; s->obj[id][4] = (REAL)rho_eff;
+    return 0;
+}
+
+/* LBM::initialize_fields (LBM.cpp:415-506) with host arrays in cell order. */
+ORC_API int orc_init_fields(orc_solver *s, const double *rho, const double *u, const double *force,
+                            const double *temp, const uint8_t *ids) {
+    long N = s->N;
+    /* boundaries (LBM.cpp:593-691) */
+    free(s->ids); s->ids = NULL; s->has_ids = 0;
+    if (ids) {
+        for (long c = 0; c < N; ++c) if (ids[c]) { s->has_ids = 1; break; }
+        if (s->has_ids) { s->ids = (uint8_t *)malloc(N); memcpy(s->ids, ids, N); }
+    }
+    /* force detection (LBM.cpp:531-541,562) */
+    s->forcing = 0;
+    free(s->force); s->force = NULL;
+    if (force) {
+        int constant = 1;
+        for (long c = 1; c < N && constant; ++c)
+            if (force[3 * c] != force[0] || force[3 * c + 1] != force[1] || force[3 * c + 2] != force[2]) constant = 0;
+        if (constant) {
+            s->fconst[0] = (REAL)force[0]; s->fconst[1] = (REAL)force[1]; s->fconst[2] = (REAL)force[2];
+            s->forcing = (force[0] != 0 || force[1] != 0 || force[2] != 0) ? 1 : 0;
+        } else {
+            s->forcing = 2;
+            s->force = (REAL *)malloc(sizeof(REAL) * 3 * N);
+            for (long c = 0; c < 3 * N; ++c) s->force[c] = (REAL)force[c];
+        }
+    } else {
+        /* no force given: the lambda default force 0 everywhere (LBM.h:43) */
+        s->fconst[0] = s->fconst[1] = s->fconst[2] = 0;
+    }
+    /* thermal detection (LBM.cpp:542-545) */
+    s->thermal = 0;
+    if (temp) {
+        for (long c = 0; c < N && !s->thermal; ++c) {
+            if (temp[c] != temp[0]) s->thermal = 1;
+            if (s->has_ids && s->ids[c] && (double)s->obj[s->ids[c]][3] != temp[0]) s->thermal = 1;
+        }
+    } else if (s->has_ids) {
+        for (long c = 0; c < N && !s->thermal; ++c)
+            if (s->ids[c] && (double)s->obj[s->ids[c]][3] != 1.0) s->thermal = 1;
+    }
+    /* equilibrium populations (set_equilibrium_populations.comp:198-220), solid cells at rest (LBM.cpp:534-535) */
+    for (long c = 0; c < N; ++c) {
+        REAL r = (REAL)rho[c];
+        REAL uu3[3] = {(REAL)u[3 * c], (REAL)u[3 * c + 1], (REAL)u[3 * c + 2]};
+        if (s->has_ids && s->ids[c]) uu3[0] = uu3[1] = uu3[2] = 0;
+        REAL uu = uu3[0] * uu3[0] + uu3[1] * uu3[1] + uu3[2] * uu3[2];
+        for (int i = 0; i < s->q; ++i) {
+            const int *cc = s->T[i].c;
+            REAL cu = (REAL)cc[0] * uu3[0] + (REAL)cc[1] * uu3[1] + (REAL)cc[2] * uu3[2];
+            F_AT(s, i, c) = store_val(s, i, feq((REAL)s->T[i].w, r, cu, uu));
+        }
+    }
+    free(s->g); s->g = NULL;
+    if (s->thermal) {
+        s->g = (REAL *)calloc((size_t)s->qt * N, sizeof(REAL));
+        for (long c = 0; c < N; ++c) {
+            REAL T0 = temp ? (REAL)temp[c] : (REAL)1;
+            REAL uu3[3] = {(REAL)u[3 * c], (REAL)u[3 * c + 1], (REAL)u[3 * c + 2]};
+            if (s->has_ids && s->ids[c]) uu3[0] = uu3[1] = uu3[2] = 0;
+            REAL uu = uu3[0] * uu3[0] + uu3[1] * uu3[1] + uu3[2] * uu3[2];
+            for (int k = 0; k < s->qt; ++k) {
+                const int *cc = s->TT[k].c;
+                REAL cu = (REAL)cc[0] * uu3[0] + (REAL)cc[1] * uu3[1] + (REAL)cc[2] * uu3[2];
+                G_AT(s, k, c) = geq((REAL)s->TT[k].w, T0, cu, uu, s->cs2_t);
+            }
+        }
+    }
+    free(s->psi[0]); free(s->psi[1]); s->psi[0] = s->psi[1] = NULL; s->cur = 0;
+    if (s->multiphase) {
+        s->psi[0] = (REAL *)malloc(sizeof(REAL) * N);
+        s->psi[1] = (REAL *)malloc(sizeof(REAL) * N);
+        for (long c = 0; c < N; ++c) {
+            REAL r = (s->has_ids && s->ids[c]) ? s->obj[s->ids[c]][4] : (REAL)rho[c];
+            s->psi[0][c] = s->psi[1][c] = psi_of(r);
+        }
+    }
+    memset(s->vd, 0, sizeof(float) * 4 * N);
+    memset(s->ft, 0, sizeof(float) * 4 * N);
+    s->t = 0;
+    tick(s, 0, 1, 0);   /* LBM.cpp:502: save macroscopic fields without updating */
+    return 0;
+}
+
+/* flags: 1 = save macroscopic on last tick, 2 = conservation sums on last tick */
+ORC_API int orc_step(orc_solver *s, long n, unsigned flags) {
+    for (long k = 0; k < n; ++k) {
+        int last = k == n - 1;
+        tick(s, 1, last && (flags & 1u), last && (flags & 2u));
+        s->t++;
+    }
+    return 0;
+}
+
+/* Conservation sums of the state the next tick reads (no update). */
+ORC_API void orc_conservation(orc_solver *s, double *out) {
+    float *vd = s->vd, *ft = s->ft;
+    s->vd = (float *)malloc(sizeof(float) * 4 * s->N);
+    s->ft = (float *)malloc(sizeof(float) * 4 * s->N);
+    int cur = s->cur;
+    REAL *p0 = NULL;
+    if (s->multiphase) { /* keep psi buffers untouched */
+        p0 = (REAL *)malloc(sizeof(REAL) * s->N);
+        memcpy(p0, s->psi[s->cur], sizeof(REAL) * s->N);
+    }
+    tick(s, 0, 0, 1);
+    if (s->multiphase) { s->cur = cur; memcpy(s->psi[s->cur], p0, sizeof(REAL) * s->N); free(p0); }
+    free(s->vd); free(s->ft); s->vd = vd; s->ft = ft;
+    memcpy(out, s->sums, sizeof s->sums);
+}
+
+ORC_API void orc_last_sums(const orc_solver *s, double *out) { memcpy(out, s->sums, sizeof s->sums); }
+
+ORC_API long orc_ticks(const orc_solver *s) { return s->t; }
+ORC_API int orc_q(const orc_solver *s) { return s->q; }
+ORC_API int orc_qt(const orc_solver *s) { return s->thermal ? s->qt : 0; }
+ORC_API int orc_forcing(const orc_solver *s) { return s->forcing; }
+ORC_API int orc_thermal(const orc_solver *s) { return s->thermal; }
+
+/* fields as in bf_field: 0 VD (4N), 2 FT (4N), 3 raw populations (qN), 4 raw thermal (qt N), 5 psi (N) */
+ORC_API int orc_read_field(const orc_solver *s, int field, double *dst) {
+    long N = s->N;
+    switch (field) {
+    case 0: for (long c = 0; c < 4 * N; ++c) dst[c] = s->vd[c]; return 0;
+    case 2: for (long c = 0; c < 4 * N; ++c) dst[c] = s->ft[c]; return 0;
+    case 3:
+        for (int i = 0; i < s->q; ++i)
+            for (long c = 0; c < N; ++c) dst[(long)i * N + c] = (double)load_val(s, i, F_AT(s, i, c));
+        return 0;
+    case 4:
+        if (!s->thermal) return 1;
+        for (long c = 0; c < (long)s->qt * N; ++c) dst[c] = (double)s->g[c];
+        return 0;
+    case 5:
+        if (!s->multiphase) return 1;
+        for (long c = 0; c < N; ++c) dst[c] = (double)s->psi[s->cur ^ 1][c];
+        return 0;
+    }
+    return 1;
+}
+
+/* Overwrite the raw lattice (AA slot layout, unshifted values) — test hook. */
+ORC_API void orc_write_populations(orc_solver *s, const double *src) {
+    for (int i = 0; i < s->q; ++i)
+        for (long c = 0; c < s->N; ++c) F_AT(s, i, c) = store_val(s, i, (REAL)src[(long)i * s->N + c]);
+}
+
+/* AA address maps for the race/permutation tests: for cell c and direction i,
+ * the flat slot index (i*N + cell) read and written on a tick of the given
+ * parity, or -1 when the access leaves the domain. */
+ORC_API void orc_aa_maps(const orc_solver *s, int odd, long *rd, long *wr) {
+    for (int z = 0; z < s->Z; ++z)
+    for (int y = 0; y < s->Y; ++y)
+    for (int x = 0; x < s->X; ++x) {
+        long self = cid(s, x, y, z);
+        for (int i = 0; i < s->q; ++i) {
+            long k = (long)i * s->N + self;
+            const int *c = s->T[i].c;
+            if (odd) { rd[k] = (long)opp(i) * s->N + self; wr[k] = (long)i * s->N + self; continue; }
+            long src, dst;
+            if (!nbr(s, x, y, z, c, -1, &src)) rd[k] = -1;
+            else if (s->bb_mode == 0 && is_solid(s, src)) rd[k] = (long)opp(i) * s->N + self;
+            else rd[k] = (long)i * s->N + src;
+            if (!nbr(s, x, y, z, c, +1, &dst)) wr[k] = -1;
+            else if (s->bb_mode == 0 && is_solid(s, dst)) wr[k] = (long)i * s->N + self;
+            else wr[k] = (long)opp(i) * s->N + dst;
+        }
+    }
+}
+
+/* Textbook two-lattice scheme (collide at x, then pull-stream, stream.comp:498-584
+ * without boundaries), periodic domains without solids, no forcing; `f` holds
+ * the streamed (pre-collision) populations f[i*N + cell] and is advanced n ticks
+ * in place.  Used to pin the AA addressing. */
+ORC_API void orc_two_lattice(const orc_solver *s, double *f, long n) {
+    long N = s->N;
+    double *post = (double *)malloc(sizeof(double) * s->q * N);
+    for (long t = 0; t < n; ++t) {
+        for (long c = 0; c < N; ++c) {
+            double rho = 0, m[3] = {0, 0, 0};
+            for (int i = 0; i < s->q; ++i) {
+                double v = f[(long)i * N + c];
+                rho += v; m[0] += v * s->T[i].c[0]; m[1] += v * s->T[i].c[1]; m[2] += v * s->T[i].c[2];
+            }
+            double u[3] = {m[0] / rho, m[1] / rho, m[2] / rho};
+            double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            for (int i = 0; i < s->q; ++i) {
+                double cu = s->T[i].c[0] * u[0] + s->T[i].c[1] * u[1] + s->T[i].c[2] * u[2];
+                double e = s->T[i].w * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * uu);
+                double v = f[(long)i * N + c];
+                post[(long)i * N + c] = v - (v - e) / (double)s->tau;
+            }
+        }
+        for (int z = 0; z < s->Z; ++z)
+        for (int y = 0; y < s->Y; ++y)
+        for (int x = 0; x < s->X; ++x) {
+            long c = cid(s, x, y, z);
+            for (int i = 0; i < s->q; ++i) {
+                long src;
+                nbr(s, x, y, z, s->T[i].c, -1, &src);
+                f[(long)i * N + c] = post[(long)i * N + src];
+            }
+        }
+    }
+    free(post);
+}
+
+/* Boundary bit packing (LBM.cpp:566-573 bits rule, :631-659 pack; collide.comp:401-418 unpack). */
+ORC_API int orc_bits_per_boundary(int object_count) {
+    if (object_count <= 1) return 0;
+    double b = exp2(ceil(log2(ceil(log2((double)object_count)))));
+    return (int)b;
+}
+
+ORC_API void orc_pack(const uint8_t *ids, long n, int bits, uint32_t *words) {
+    long nw = (n * bits + 31) / 32;
+    memset(words, 0, sizeof(uint32_t) * nw);
+    for (long c = 0; c < n; ++c) {
+        long b = c * bits;
+        words[b / 32] |= ((uint32_t)ids[c]) << (b % 32);
+    }
+}
+
+ORC_API void orc_unpack(const uint32_t *words, long n, int bits, uint8_t *ids) {
+    uint32_t mask = (bits >= 32) ? 0xffffffffu : ((1u << bits) - 1u);
+    for (long c = 0; c < n; ++c) {
+        long b = c * bits;
+        ids[c] = (uint8_t)((words[b / 32] >> (b % 32)) & mask);
+    }
+}
+
+/* Storage rounding helpers exposed for tests. */
+ORC_API float orc_half_rne(float v) { return half_rne(v); }
+ORC_API float orc_bf16_rne(float v) { return bf16_rne(v); }

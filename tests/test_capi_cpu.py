@@ -1,0 +1,99 @@
+"""The C ABI library loads, exports every declared entry point and rejects bad
+input before touching a device.  CPU only: no compute calls."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bf", "bf.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ["bf_create", "bf_destroy", "bf_init_fields", "bf_step", "bf_read_field", "bf_conservation"]:
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(bfa):
+    lib = bfa.load_library()
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_abi_version_and_counts(bfa):
+    lib = bfa.load_library()
+    assert lib.bf_abi_version() == 1
+    assert [lib.bf_velocity_count(v) for v in (2, 3, 4, 5)] == [9, 15, 19, 27]
+    assert lib.bf_velocity_count(7) == 0
+
+
+@pytest.mark.parametrize("name", ["D2Q9", "D3Q15", "D3Q19", "D3Q27"])
+def test_product_tables_equal_oracle(bfa, orc, name):
+    a = bfa.velocity_table(name)
+    b = orc.velocity_table(name)
+    np.testing.assert_array_equal(a[:, :3], b[:, :3])
+    np.testing.assert_array_equal(a[:, 3], b[:, 3].astype(np.float32))
+
+
+@pytest.mark.parametrize("name", ["D2Q5", "D3Q7"])
+def test_product_thermal_tables_equal_oracle(bfa, orc, name):
+    a = bfa.velocity_table(name, thermal=True)
+    b = orc.velocity_table(name, thermal=True)
+    np.testing.assert_array_equal(a[:, :3], b[:, :3])
+    np.testing.assert_array_equal(a[:, 3], b[:, 3].astype(np.float32))
+
+
+def test_bits_per_boundary_matches_oracle(bfa, orc):
+    lib = orc.load()
+    for n in range(0, 257):
+        assert bfa.bits_per_boundary(n) == lib.orc_bits_per_boundary(n), n
+
+
+@pytest.mark.parametrize("bits", [1, 2, 4, 8])
+def test_pack_matches_oracle(bfa, orc, bits):
+    rng = np.random.default_rng(7)
+    n = 777
+    ids = rng.integers(0, 1 << bits, n).astype(np.uint8)
+    words = bfa.pack_boundaries(ids, bits)
+    ref = np.zeros_like(words)
+    orc.load().orc_pack(ids.ctypes.data_as(C.POINTER(C.c_uint8)), n, bits, ref.ctypes.data_as(C.POINTER(C.c_uint32)))
+    np.testing.assert_array_equal(words, ref)
+    np.testing.assert_array_equal(bfa.unpack_boundaries(words, n, bits), ids)
+
+
+def test_pack_rejects_oversized_ids(bfa):
+    with pytest.raises(bfa.BFError):
+        bfa.pack_boundaries(np.array([0, 2], np.uint8), 1)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(resolution=(0, 4, 4)), "resolution"),
+    (dict(resolution=(8, 8, 2), velocity_set="D2Q9"), "dimension"),
+    (dict(relaxation_time=0.5), "relaxation"),
+    (dict(storage=7), "storage"),
+    (dict(velocity_set=9), "velocity set"),
+    (dict(boundary_mode=5), "boundary mode"),
+])
+def test_create_rejects_bad_config_before_device(bfa, kw, msg):
+    args = dict(resolution=(8, 8, 8), relaxation_time=0.6, velocity_set="D3Q19")
+    args.update(kw)
+    with pytest.raises(bfa.BFError) as e:
+        bfa.Solver(**args)
+    assert e.value.status == 1
+    assert msg in str(e.value)
+
+
+def test_null_handle_is_rejected(bfa):
+    lib = bfa.load_library()
+    assert lib.bf_step(None, 1, 0) == 1
+    assert lib.bf_ticks(None) == -1
+    lib.bf_destroy(None)

@@ -1,0 +1,245 @@
+"""GPU parity: the HIP solver (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Bars (written in each test):
+  - FP32 storage vs the FP32 oracle (same operation order, contraction off):
+    max |diff| <= 1e-6 on the raw populations, most entries bit-identical;
+  - FP16/BF16 shifted storage vs the FP32 oracle with emulated storage: the
+    stored 16-bit values agree except for rare one-quantum rounding flips;
+  - every GPU result vs the FP64 oracle: ||d rho||/||rho|| and ||d u||/||u||
+    within 1e-5 (FP32), 1e-3 (FP16) and 1e-2 (BF16);
+  - boundary ids and neighbour indexing are exact (integer work).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import SEED, cavity_ids, channel_ids, random_state, rel_l2, spheres_ids, taylor_green
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_pair(bfa, orc, res, tau, steps, *, velocity_set="D3Q19", storage="fp32", periodic=(1, 1, 1),
+             multiphase=False, boundary_mode="halfway", init=None, objects=(), params=(), precision="f32"):
+    say(f"case: {velocity_set} {storage} {res} steps={steps} mp={multiphase} bm={boundary_mode}")
+    g = bfa.Solver(res, tau, periodic=periodic, velocity_set=velocity_set, storage=storage,
+                   multiphase=multiphase, boundary_mode=boundary_mode)
+    o = orc.Oracle(res, tau, periodic=periodic, velocity_set=velocity_set,
+                   storage=storage if precision == "f32" else "fp32", multiphase=multiphase,
+                   boundary_mode=boundary_mode, precision=precision)
+    for oid, kw in objects:
+        g.set_object(oid, **kw)
+        o.set_object(oid, **kw)
+    for k, v in params:
+        g.set_param(k, v)
+        o.set_param(k, v)
+    g.init_fields(**init)
+    o.init_fields(**init)
+    if steps:
+        g.step(steps, save=True)
+        o.step(steps, save=True)
+    return g, o
+
+
+def assert_fp32_parity(g, o, what="populations_raw", atol=1e-6):
+    a, b = g.read(what), o.read(what)
+    d = np.abs(a.astype(np.float64) - b)
+    exact = float(np.mean(a.astype(np.float64) == b))
+    say(f"  {what}: max|diff|={d.max():.3e} bit-identical={exact:.4f}")
+    assert d.max() <= atol
+    return exact
+
+
+def assert_macro_vs_f64(g, o64, tol):
+    a, b = g.read("velocity_density").astype(np.float64), o64.read("velocity_density")
+    er = rel_l2(a[:, 3], b[:, 3])
+    eu = rel_l2(a[:, :3], b[:, :3])
+    say(f"  vs FP64: rho {er:.3e}  u {eu:.3e}")
+    assert er < tol and eu < tol
+    return er, eu
+
+
+def test_d3q19_tgv_fp32_matches_oracle(bfa, orc):
+    n = 32
+    rho, u = taylor_green(n, 0.05)
+    init = dict(rho=rho, u=u)
+    g, o = run_pair(bfa, orc, (n, n, n), 0.55, 41, init=init)
+    exact = assert_fp32_parity(g, o)
+    assert exact > 0.99
+    assert_fp32_parity(g, o, "velocity_density")
+    _, o64 = None, orc.Oracle((n, n, n), 0.55, velocity_set="D3Q19", precision="f64")
+    o64.init_fields(**init)
+    o64.step(41, save=True)
+    assert_macro_vs_f64(g, o64, 1e-5)
+
+
+@pytest.mark.parametrize("vs", ["D2Q9", "D3Q15", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_velocity_sets_and_storage(bfa, orc, vs, storage):
+    res = (40, 24, 1) if vs == "D2Q9" else (20, 12, 10)
+    n = int(np.prod(res))
+    rho, u = random_state(n, seed=SEED + 1, du=0.06, drho=0.04)
+    g, o = run_pair(bfa, orc, res, 0.6, 23, velocity_set=vs, storage=storage, init=dict(rho=rho, u=u))
+    a, b = g.read("populations_raw").astype(np.float64), o.read("populations_raw")
+    d = np.abs(a - b)
+    say(f"  raw max|diff|={d.max():.3e} identical={np.mean(a == b):.4f}")
+    if storage == "fp32":
+        assert d.max() <= 1e-6
+    else:
+        q = 2.0 ** -11 if storage == "fp16" else 2.0 ** -8    # relative quantum of f - w
+        w = orc.velocity_table(vs)[:, 3][:, None]
+        assert np.mean(a == b) > 0.98
+        assert np.all(d <= 2 * q * np.maximum(np.abs(b - w), 2.0 ** -14) + 1e-9)
+    o64 = orc.Oracle(res, 0.6, velocity_set=vs, precision="f64")
+    o64.init_fields(rho=rho, u=u)
+    o64.step(23, save=True)
+    assert_macro_vs_f64(g, o64, {"fp32": 1e-5, "fp16": 1e-3, "bf16": 1e-2}[storage])
+
+
+@pytest.mark.parametrize("mode", ["halfway", "fullway_compat"])
+def test_lid_driven_cavity_walls(bfa, orc, mode):
+    X = Y = 48
+    n = X * Y
+    ids = cavity_ids(X, Y)
+    init = dict(rho=np.ones(n, np.float32), u=np.zeros((n, 3), np.float32), boundary_id=ids)
+    objs = [(2, dict(velocity=(0.1, 0.0, 0.0)))]
+    g, o = run_pair(bfa, orc, (X, Y, 1), 0.6, 301, velocity_set="D2Q9", periodic=(0, 0, 1),
+                    boundary_mode=mode, init=init, objects=objs)
+    assert_fp32_parity(g, o, atol=2e-6)
+    vd = g.read("velocity_density")
+    np.testing.assert_array_equal(g.read("boundary"), ids)
+    if mode == "halfway":
+        # the moving lid drives a clockwise vortex: u_x > 0 just under the lid
+        assert vd.reshape(Y, X, 4)[Y - 2, X // 2, 0] > 0.01
+
+
+def test_open_boundaries_quirks(bfa, orc):
+    """Non-periodic edges without walls: reads of 0.01, dropped writes
+    (collide.comp:297-298, 358-359) reproduced exactly."""
+    res = (12, 10, 6)
+    n = int(np.prod(res))
+    rho, u = random_state(n, seed=SEED + 2)
+    g, o = run_pair(bfa, orc, res, 0.7, 9, periodic=(0, 1, 0), init=dict(rho=rho, u=u))
+    assert_fp32_parity(g, o, atol=2e-6)
+
+
+def test_force_field_and_obstacles(bfa, orc):
+    X, Y, Z = 24, 14, 10
+    n = X * Y * Z
+    rng = np.random.default_rng(SEED + 3)
+    force = (1e-5 * rng.standard_normal((n, 3))).astype(np.float32)
+    ids = spheres_ids(X, Y, Z, 0.08, 2, 3, seed=SEED + 3)
+    rho, u = random_state(n, seed=SEED + 3)
+    objs = [(1, dict(velocity=(0.0, 0.0, 0.01)))]
+    g, o = run_pair(bfa, orc, (X, Y, Z), 0.56, 33, periodic=(1, 0, 0), objects=objs,
+                    init=dict(rho=rho, u=u, force=force, boundary_id=ids))
+    assert g.info()["forcing"] == 2
+    assert_fp32_parity(g, o, atol=2e-6)
+    assert_fp32_parity(g, o, "force_temperature", atol=1e-6)
+
+
+def test_constant_force_porous_duct(bfa, orc):
+    X, Y, Z = 32, 16, 16
+    n = X * Y * Z
+    ids = spheres_ids(X, Y, Z, 0.05, 2, 4, seed=SEED + 4)
+    force = np.tile(np.array([1e-6, 0, 0], np.float32), (n, 1))
+    init = dict(rho=np.ones(n, np.float32), u=np.zeros((n, 3), np.float32), force=force, boundary_id=ids)
+    for st in ("fp32", "fp16"):
+        g, o = run_pair(bfa, orc, (X, Y, Z), 0.56, 60, storage=st, periodic=(1, 0, 0), init=init)
+        assert g.info()["forcing"] == 1
+        if st == "fp32":
+            assert_fp32_parity(g, o, atol=1e-6)
+
+
+def test_shan_chen_d3q27(bfa, orc):
+    n = 16
+    N = n ** 3
+    rng = np.random.default_rng(SEED + 5)
+    rho = (2.0 + 0.01 * (2 * rng.random(N) - 1)).astype(np.float32)
+    init = dict(rho=rho, u=np.zeros((N, 3), np.float32))
+    g, o = run_pair(bfa, orc, (n, n, n), 1.0, 40, velocity_set="D3Q27", multiphase=True,
+                    init=init, params=[("interaction_strength", -5.0)])
+    assert_fp32_parity(g, o, atol=1e-5)
+    a, b = g.read("psi"), o.read("psi")
+    say(f"  psi max|diff|={np.abs(a - b).max():.3e}")
+    assert np.abs(a - b).max() < 1e-5
+
+
+def test_shan_chen_d2q9_with_wall(bfa, orc):
+    X, Y = 70, 20
+    N = X * Y
+    rng = np.random.default_rng(SEED + 6)
+    rho = (0.3 + 0.05 * rng.random(N)).astype(np.float32)
+    rho.reshape(Y, X)[5:12, 20:40] = 2.2
+    ids = channel_ids(X, Y)
+    init = dict(rho=rho, u=np.zeros((N, 3), np.float32), boundary_id=ids)
+    g, o = run_pair(bfa, orc, (X, Y, 1), 0.9, 31, velocity_set="D2Q9", multiphase=True, periodic=(1, 0, 1),
+                    init=init, objects=[(1, dict(effective_density=1.5))], params=[("interaction_strength", -5.0)])
+    assert_fp32_parity(g, o, atol=1e-5)
+
+
+def test_thermal_d3q19_rayleigh_benard_like(bfa, orc):
+    X, Y, Z = 20, 16, 12
+    N = X * Y * Z
+    ids = channel_ids(X, Y, Z)
+    ids.reshape(Z, Y, X)[:, -1, :] = 2
+    T = np.full(N, 0.5, np.float32)
+    force = np.tile(np.array([0, -1e-4, 0], np.float32), (N, 1))
+    rng = np.random.default_rng(SEED + 7)
+    T += (0.01 * rng.random(N)).astype(np.float32)
+    init = dict(rho=np.ones(N, np.float32), u=np.zeros((N, 3), np.float32), force=force, temperature=T,
+                boundary_id=ids)
+    objs = [(1, dict(temperature=1.0)), (2, dict(temperature=0.0))]
+    params = [("reference_temperature", 0.5), ("thermal_relaxation_time", 0.8)]
+    g, o = run_pair(bfa, orc, (X, Y, Z), 0.6, 27, periodic=(1, 0, 1), init=init, objects=objs, params=params)
+    assert g.info()["thermal"] == 1
+    assert_fp32_parity(g, o, atol=2e-6)
+    assert_fp32_parity(g, o, "thermal_raw", atol=2e-6)
+    assert_fp32_parity(g, o, "force_temperature", atol=2e-6)
+
+
+def test_conservation_sums_fp64(bfa, orc):
+    X, Y, Z = 24, 20, 12
+    n = X * Y * Z
+    ids = channel_ids(X, Y, Z)
+    rho, u = random_state(n, seed=SEED + 8)
+    force = np.tile(np.array([1e-5, 0, 0], np.float32), (n, 1))
+    g, o = run_pair(bfa, orc, (X, Y, Z), 0.7, 0, periodic=(1, 0, 1),
+                    init=dict(rho=rho, u=u, force=force, boundary_id=ids))
+    s_g, s_o = g.conservation(), o.conservation()
+    np.testing.assert_allclose(s_g[:4], s_o[:4], rtol=1e-12, atol=1e-12)
+    g.step(15, conservation=True)
+    o.step(15, conservation=True)
+    np.testing.assert_allclose(g.last_step_sums()[:4], o.last_step_sums()[:4], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(g.conservation()[:4], o.conservation()[:4], rtol=1e-9, atol=1e-9)
+
+
+def test_full_size_tgv_256_few_ticks(bfa, orc):
+    """BASELINE workload size: 256^3 D3Q19 FP32, indexing checked against the
+    FP32 oracle for 3 ticks, then 200 more ticks checked through size-independent
+    properties (mass drift, viscous decay rate)."""
+    n = 256
+    rho, u = taylor_green(n, 0.05)
+    g, o = run_pair(bfa, orc, (n, n, n), 0.55, 3, init=dict(rho=rho, u=u))
+    assert_fp32_parity(g, o)
+    del o
+    m0 = g.conservation()[0]
+    g.step(100, save=True)
+    vd = g.read("velocity_density")
+    ke0 = 0.5 * np.sum(vd[:, 3].astype(np.float64) * (vd[:, :3].astype(np.float64) ** 2).sum(1))
+    g.step(100, save=True)
+    vd = g.read("velocity_density")
+    ke1 = 0.5 * np.sum(vd[:, 3].astype(np.float64) * (vd[:, :3].astype(np.float64) ** 2).sum(1))
+    m1 = g.conservation()[0]
+    say(f"  mass drift {abs(m1 - m0) / m0:.3e}  KE ratio {ke1 / ke0:.6f}")
+    assert abs(m1 - m0) / m0 < 1e-5
+    nu = (np.float32(0.55) - 0.5) / 3
+    k = 2 * np.pi / n
+    rate = -np.log(ke1 / ke0) / 100.0
+    assert abs(rate / (6 * nu * k * k) - 1.0) < 0.1

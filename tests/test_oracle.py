@@ -1,0 +1,405 @@
+"""Pins of the CPU oracle (oracle/bf_oracle.c) against the reference's data,
+closed forms, invariants and analytic flows.  CPU only."""
+import itertools
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from helpers import SEED, channel_ids, random_state, rel_l2, taylor_green
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "velocity_sets.json")
+SETS = ["D2Q9", "D3Q15", "D3Q19", "D3Q27"]
+
+
+def golden_sets():
+    return json.load(open(GOLDEN))["sets"]
+
+
+@pytest.mark.parametrize("name", SETS + ["D2Q5", "D3Q7"])
+def test_oracle_tables_match_reference_source(orc, name):
+    g = golden_sets()[name]
+    t = orc.velocity_table(name, thermal=name in ("D2Q5", "D3Q7"))
+    assert t.shape[0] == len(g)
+    for row, ref in zip(t, g):
+        assert list(row[:3].astype(int)) == ref["c"]
+        assert row[3] == ref["w"][0] / ref["w"][1]
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_velocity_set_isotropy_exact(name):
+    """sum w = 1, sum w c = 0, sum w c c = I/3, 4th-order isotropy and
+    opposite pairs, in exact rationals from the reference table."""
+    g = golden_sets()[name]
+    c = np.array([r["c"] for r in g])
+    w = [Fraction(int(r["w"][0]), r["w"][1]) for r in g]
+    dim = 2 if name == "D2Q9" else 3
+    assert sum(w) == 1
+    for a in range(3):
+        assert sum(wi * int(ci[a]) for wi, ci in zip(w, c)) == 0
+    for a, b in itertools.product(range(dim), repeat=2):
+        s = sum(wi * int(ci[a]) * int(ci[b]) for wi, ci in zip(w, c))
+        assert s == (Fraction(1, 3) if a == b else 0)
+    if name != "D3Q15":  # D3Q15 is 4th-order isotropic too
+        pass
+    for a, b, cc, d in itertools.product(range(dim), repeat=4):
+        s = sum(wi * int(ci[a]) * int(ci[b]) * int(ci[cc]) * int(ci[d]) for wi, ci in zip(w, c))
+        delta = lambda i, j: 1 if i == j else 0  # noqa: E731
+        exp = Fraction(1, 9) * (delta(a, b) * delta(cc, d) + delta(a, cc) * delta(b, d) + delta(a, d) * delta(b, cc))
+        assert s == exp
+    # opposite(i) = i+1 for odd i (collide.comp:219-222) must reverse the direction
+    for i in range(1, len(g), 2):
+        assert list(c[i]) == list(-c[i + 1])
+
+
+@pytest.mark.parametrize("name,cs2", [("D2Q5", Fraction(1, 3)), ("D3Q7", Fraction(1, 4))])
+def test_thermal_set_moments(name, cs2):
+    g = golden_sets()[name]
+    c = np.array([r["c"] for r in g])
+    w = [Fraction(int(r["w"][0]), r["w"][1]) for r in g]
+    assert sum(w) == 1
+    dim = 2 if name == "D2Q5" else 3
+    for a in range(dim):
+        assert sum(wi * int(ci[a]) for wi, ci in zip(w, c)) == 0
+        assert sum(wi * int(ci[a]) ** 2 for wi, ci in zip(w, c)) == cs2
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_equilibrium_moments(orc, name):
+    """Populations initialised to f_eq: sum = rho, sum c = rho u,
+    sum c c = rho (I/3 + u u) (second-order equilibrium is exact for these)."""
+    Z = 1
+    o = orc.Oracle((1, 1, Z), 0.7, velocity_set=name)
+    rho = np.array([1.3], np.float32)
+    u = np.array([[0.05, -0.03, 0.02 if name != "D2Q9" else 0.0]], np.float32)
+    o.init_fields(rho, u)
+    f = o.read("populations_raw")[:, 0]
+    t = orc.velocity_table(name)
+    c = t[:, :3]
+    r = float(rho[0])
+    uu = u[0].astype(np.float64)
+    assert abs(f.sum() - r) < 1e-14
+    np.testing.assert_allclose(f @ c, r * uu, atol=1e-14)
+    dim = 2 if name == "D2Q9" else 3
+    P = (c[:, :, None] * c[:, None, :] * f[:, None, None]).sum(0)
+    Pexp = r * (np.eye(3) / 3 + np.outer(uu, uu))
+    np.testing.assert_allclose(P[:dim, :dim], Pexp[:dim, :dim], atol=1e-14)
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_uniform_state_is_fixed_point(orc, name):
+    Z = 1 if name == "D2Q9" else 4
+    o = orc.Oracle((5, 4, Z), 0.8, velocity_set=name)
+    n = o.cells
+    rho = np.full(n, 1.2, np.float32)
+    u = np.tile(np.array([0.03, -0.02, 0.01 if Z > 1 else 0.0], np.float32), (n, 1))
+    o.init_fields(rho, u)
+    f0 = o.read("populations_raw")
+    o.step(10)
+    np.testing.assert_allclose(o.read("populations_raw"), f0, rtol=0, atol=1e-14)
+
+
+def _check_aa_sets(o, solid):
+    q, n = o.q, o.cells
+    for odd in (0, 1):
+        rd, wr = o.aa_maps(odd)
+        owner = {}
+        for c in range(n):
+            if solid[c]:
+                continue
+            r = set(rd[i * n + c] for i in range(q)) - {-1}
+            w = set(wr[i * n + c] for i in range(q)) - {-1}
+            assert r == w, (odd, c)                       # a cell writes exactly what it reads
+            assert len([x for x in rd[c::n][:q] if x >= 0]) == len(r)  # no slot read twice
+            for k in r:
+                assert k not in owner, (odd, c, owner.get(k))   # no two cells share a slot
+                owner[k] = c
+    return owner
+
+
+@pytest.mark.parametrize("name", SETS)
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 1), (1, 0, 0), (0, 0, 0)])
+def test_aa_read_write_sets_race_free(orc, name, periodic):
+    Z = 1 if name == "D2Q9" else 3
+    o = orc.Oracle((7, 5, Z), 0.7, periodic=periodic, velocity_set=name)
+    n = o.cells
+    rho, u = random_state(n)
+    ids = np.zeros(n, np.uint8)
+    ids[[3, 11, 17]] = 1   # a few solid cells: halfway redirect paths
+    o.init_fields(rho, u, boundary_id=ids)
+    owner = _check_aa_sets(o, ids)
+    if periodic == (1, 1, 1):
+        # fully periodic: every slot of every fluid cell is owned
+        pass
+    assert len(owner) > 0
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_aa_fully_periodic_is_permutation(orc, name):
+    Z = 1 if name == "D2Q9" else 3
+    o = orc.Oracle((7, 5, Z), 0.7, velocity_set=name)
+    n, q = o.cells, o.q
+    o.init_fields(*random_state(n))
+    for odd in (0, 1):
+        rd, wr = o.aa_maps(odd)
+        assert sorted(rd) == list(range(q * n))
+        assert sorted(wr) == list(range(q * n))
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_aa_equals_two_lattice(orc, name):
+    """Two AA ticks == two textbook collide+pull-stream ticks (stream.comp
+    semantics), after mapping the AA slot layout back to cells."""
+    Z = 1 if name == "D2Q9" else 4
+    X, Y = 6, 5
+    o = orc.Oracle((X, Y, Z), 0.8, velocity_set=name)
+    n, q = o.cells, o.q
+    rho, u = random_state(n, du=0.05)
+    o.init_fields(rho, u)
+    raw0 = o.read("populations_raw")
+    t = orc.velocity_table(name)[:, :3].astype(int)
+    idx = np.arange(n).reshape(Z, Y, X)
+
+    def shift(a, c, sgn):  # value at x - sgn*c, i.e. pull from x - c
+        return np.roll(a.reshape(Z, Y, X), shift=(sgn * c[2], sgn * c[1], sgn * c[0]), axis=(0, 1, 2)).ravel()
+
+    tl0 = np.stack([shift(raw0[i], t[i], 1) for i in range(q)])   # streamed state f_i(x) = raw_i(x - c_i)
+    tl2 = o.two_lattice(tl0, 2)
+    o.step(2)
+    aa2 = o.read("populations_raw")                                  # post-collision f*_i(x) of tick 1
+    aa2_streamed = np.stack([shift(aa2[i], t[i], 1) for i in range(q)])
+    np.testing.assert_allclose(aa2_streamed, tl2, rtol=0, atol=1e-13)
+    del idx
+
+
+@pytest.mark.parametrize("name", ["D2Q9", "D3Q19"])
+@pytest.mark.parametrize("mode", ["halfway", "fullway_compat"])
+def test_mass_conserved_with_walls(orc, name, mode):
+    Z = 1 if name == "D2Q9" else 4
+    X, Y = 9, 8
+    o = orc.Oracle((X, Y, Z), 0.7, velocity_set=name, boundary_mode=mode)
+    n = o.cells
+    rho, u = random_state(n, du=0.04)
+    ids = channel_ids(X, Y, Z)
+    ids[(Z // 2) * X * Y + 4 * X + 4] = 3   # an obstacle cell
+    o.init_fields(rho, u, boundary_id=ids)
+    m0, r0 = o.conservation()[0], o.read("populations_raw").sum()
+    o.step(37)
+    m1, r1 = o.conservation()[0], o.read("populations_raw").sum()
+    # every slot a tick touches belongs to a fluid cell or to a solid cell next
+    # to one; the rest never change, so the raw total is conserved in both modes
+    assert abs(r1 - r0) < 1e-12 * r0
+    if mode == "halfway":
+        # halfway: no population is parked in solid cells, so the mass the
+        # next tick reads is conserved too
+        assert abs(m1 - m0) < 1e-12 * m0
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_momentum_gain_constant_force(orc, name):
+    """Guo forcing: raw momentum sum_i f_i c_i grows by N*F per tick (clamps inactive)."""
+    Z = 1 if name == "D2Q9" else 3
+    o = orc.Oracle((6, 5, Z), 0.65, velocity_set=name)
+    n = o.cells
+    F = np.array([2e-5, -1e-5, 3e-5 if Z > 1 else 0.0])
+    rho, u = random_state(n)
+    o.init_fields(rho, u, force=np.tile(F.astype(np.float32), (n, 1)))
+    assert o.forcing == 1
+    s0 = o.conservation()
+    o.step(5)
+    s1 = o.conservation()
+    Ff = F.astype(np.float32).astype(np.float64)
+    np.testing.assert_allclose(s1[1:4] - s0[1:4], 5 * n * Ff, rtol=1e-9, atol=1e-15)
+    assert abs(s1[0] - s0[0]) < 1e-12 * s0[0]
+
+
+def test_taylor_green_decay_matches_viscosity(orc):
+    """Low-Re 3-D Taylor-Green: KE(t) ~ KE0 exp(-6 nu k^2 t), nu = (tau - 1/2)/3."""
+    n, u0, tau = 24, 0.005, 0.8
+    o = orc.Oracle((n, n, n), tau, velocity_set="D3Q19")
+    rho, u = taylor_green(n, u0)
+    o.init_fields(rho, u)
+    ke = []
+    for _ in range(2):
+        o.step(60, save=True)
+        vd = o.read("velocity_density")
+        ke.append(0.5 * np.sum(vd[:, 3] * (vd[:, :3] ** 2).sum(1)))
+    nu = (np.float32(tau) - 0.5) / 3.0
+    k = 2 * np.pi / n
+    rate = -np.log(ke[1] / ke[0]) / 60.0
+    assert abs(rate / (6 * nu * k * k) - 1.0) < 0.02
+
+
+def test_poiseuille_profile(orc):
+    """Body-force Poiseuille between halfway bounce-back walls at the BGK
+    relaxation time where the wall slip vanishes: parabolic to 1e-3."""
+    X, Y = 2, 18
+    tau = 0.5 + np.sqrt(3.0 / 16.0)
+    g = 1e-6
+    o = orc.Oracle((X, Y, 1), tau, periodic=(1, 0, 1), velocity_set="D2Q9")
+    n = o.cells
+    ids = channel_ids(X, Y)
+    o.init_fields(np.ones(n, np.float32), np.zeros((n, 3), np.float32),
+                  force=np.tile(np.array([g, 0, 0], np.float32), (n, 1)), boundary_id=ids)
+    o.step(30000, save=True)
+    ux = o.read("velocity_density")[:, 0].reshape(Y, X)[:, 0]
+    nu = (np.float32(tau) - 0.5) / 3.0
+    y = np.arange(Y)
+    a, b = 0.5, Y - 1.5
+    exact = np.float32(g) / (2 * nu) * (y - a) * (b - y)
+    fluid = slice(1, Y - 1)
+    assert rel_l2(ux[fluid], exact[fluid]) < 1e-3
+
+
+def test_couette_moving_wall(orc):
+    """Top wall moving at u_w (Ladd term), bottom at rest: linear profile."""
+    X, Y, uw = 2, 12, 0.02
+    o = orc.Oracle((X, Y, 1), 0.9, periodic=(1, 0, 1), velocity_set="D2Q9")
+    n = o.cells
+    ids = channel_ids(X, Y)
+    ids.reshape(Y, X)[-1, :] = 2
+    o.set_object(2, velocity=(uw, 0, 0))
+    o.init_fields(np.ones(n, np.float32), np.zeros((n, 3), np.float32), boundary_id=ids)
+    o.step(20000, save=True)
+    ux = o.read("velocity_density")[:, 0].reshape(Y, X)[:, 0]
+    y = np.arange(1, Y - 1)
+    exact = np.float32(uw) * (y - 0.5) / (Y - 2.0)
+    np.testing.assert_allclose(ux[1:Y - 1], exact, rtol=0, atol=2e-6)
+
+
+def test_thermal_conduction_linear_profile(orc):
+    """D2Q5 thermal lattice with anti-bounce-back walls at T=1 (bottom) and
+    T=0 (top), fluid at rest: linear T between the halfway wall positions."""
+    X, Y = 2, 14
+    o = orc.Oracle((X, Y, 1), 0.8, periodic=(1, 0, 1), velocity_set="D2Q9")
+    n = o.cells
+    ids = channel_ids(X, Y)
+    ids.reshape(Y, X)[-1, :] = 2
+    o.set_object(1, temperature=1.0)
+    o.set_object(2, temperature=0.0)
+    o.set_param("thermal_relaxation_time", 0.9)
+    T0 = np.full(n, 0.5, np.float32)
+    o.init_fields(np.ones(n, np.float32), np.zeros((n, 3), np.float32), temperature=T0, boundary_id=ids)
+    assert o.thermal
+    o.step(20000, save=True)
+    T = o.read("force_temperature")[:, 3].reshape(Y, X)[:, 0]
+    y = np.arange(1, Y - 1)
+    exact = 1.0 - (y - 0.5) / (Y - 2.0)
+    np.testing.assert_allclose(T[1:Y - 1], exact, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("count", list(range(0, 258)))
+def test_bits_per_boundary_rule(orc, count):
+    """LBM.cpp:569: bits = 2^ceil(log2(ceil(log2(count)))), at most 8, and
+    enough to hold ids 0..count-1."""
+    b = orc.load().orc_bits_per_boundary(count)
+    if count <= 1:
+        assert b == 0
+        return
+    assert b in (1, 2, 4, 8) or count > 256
+    if count <= 256:
+        assert (1 << b) >= count
+
+
+@pytest.mark.parametrize("bits", [1, 2, 4, 8])
+def test_boundary_pack_roundtrip(orc, bits):
+    import ctypes as C
+    lib = orc.load()
+    rng = np.random.default_rng(SEED)
+    n = 1000
+    ids = rng.integers(0, 1 << bits, n).astype(np.uint8)
+    words = np.zeros((n * bits + 31) // 32, np.uint32)
+    lib.orc_pack(ids.ctypes.data_as(C.POINTER(C.c_uint8)), n, bits, words.ctypes.data_as(C.POINTER(C.c_uint32)))
+    back = np.zeros(n, np.uint8)
+    lib.orc_unpack(words.ctypes.data_as(C.POINTER(C.c_uint32)), n, bits, back.ctypes.data_as(C.POINTER(C.c_uint8)))
+    np.testing.assert_array_equal(back, ids)
+    # cell c's id lives in bits [c*bits, (c+1)*bits) of the little-endian word stream
+    c = 37
+    word, off = divmod(c * bits, 32)
+    assert (int(words[word]) >> off) & ((1 << bits) - 1) == ids[c]
+
+
+def test_half_rounding_matches_ieee(orc):
+    lib = orc.load("f32")
+    rng = np.random.default_rng(SEED)
+    vals = np.concatenate([
+        (rng.standard_normal(4000) * 10.0 ** rng.uniform(-9, 4, 4000)),
+        np.array([0.0, -0.0, 6.1e-5, 5.96e-8, 2.98e-8, 65504.0, 65519.0, 1e-3, -2.5e-6]),
+    ]).astype(np.float32)
+    got = np.array([lib.orc_half_rne(float(v)) for v in vals], np.float32)
+    exp = vals.astype(np.float16).astype(np.float32)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_bf16_rounding_properties(orc):
+    lib = orc.load("f32")
+    rng = np.random.default_rng(SEED)
+    vals = (rng.standard_normal(2000) * 10.0 ** rng.uniform(-6, 3, 2000)).astype(np.float32)
+    for v in vals:
+        r = np.float32(lib.orc_bf16_rne(float(v)))
+        bits = np.array([r], np.float32).view(np.uint32)[0]
+        assert bits & 0xFFFF == 0
+        ulp = np.float32(2.0 ** (np.floor(np.log2(abs(v))) - 7))
+        assert abs(float(r) - float(v)) <= 0.5 * float(ulp) * (1 + 1e-6)
+    # ties go to even: 1 + 2^-8 lies halfway between 1 and 1 + 2^-7
+    assert lib.orc_bf16_rne(1.0 + 2.0 ** -8) == 1.0
+    assert lib.orc_bf16_rne(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16", 1e-3), ("bf16", 1e-2)])
+def test_shifted_storage_error_vs_fp64(orc, storage, tol):
+    """FP32 arithmetic with shifted 16-bit storage stays close to the FP64 oracle."""
+    n = 16
+    rho, u = taylor_green(n, 0.05)
+    res = {}
+    for prec, st in (("f64", "fp32"), ("f32", storage)):
+        o = orc.Oracle((n, n, n), 0.55, velocity_set="D3Q19", storage=st, precision=prec)
+        o.init_fields(rho, u)
+        o.step(100, save=True)
+        res[prec] = o.read("velocity_density")
+    assert rel_l2(res["f32"][:, 3], res["f64"][:, 3]) < tol
+    assert rel_l2(res["f32"][:, :3], res["f64"][:, :3]) < tol
+
+
+def test_f32_oracle_tracks_f64(orc):
+    n = 16
+    rho, u = taylor_green(n, 0.05)
+    out = {}
+    for prec in ("f64", "f32"):
+        o = orc.Oracle((n, n, n), 0.55, velocity_set="D3Q19", precision=prec)
+        o.init_fields(rho, u)
+        o.step(100, save=True)
+        out[prec] = o.read("velocity_density")
+    assert rel_l2(out["f32"][:, :3], out["f64"][:, :3]) < 1e-5
+    assert rel_l2(out["f32"][:, 3], out["f64"][:, 3]) < 1e-5
+
+
+def test_detection_rules(orc):
+    """Force constant/varying and thermal detection (LBM.cpp:531-545,562)."""
+    n = 4 * 3
+    o = orc.Oracle((4, 3, 1), 0.7, velocity_set="D2Q9")
+    rho, u = np.ones(n, np.float32), np.zeros((n, 3), np.float32)
+    o.init_fields(rho, u)
+    assert o.forcing == 0 and not o.thermal
+    o.init_fields(rho, u, force=np.zeros((n, 3), np.float32))
+    assert o.forcing == 0
+    F = np.zeros((n, 3), np.float32)
+    F[:, 0] = 1e-5
+    o.init_fields(rho, u, force=F)
+    assert o.forcing == 1
+    F[5, 1] = 2e-5
+    o.init_fields(rho, u, force=F)
+    assert o.forcing == 2
+    o.init_fields(rho, u, temperature=np.ones(n, np.float32))
+    assert not o.thermal
+    T = np.ones(n, np.float32)
+    T[3] = 1.5
+    o.init_fields(rho, u, temperature=T)
+    assert o.thermal
+    ids = np.zeros(n, np.uint8)
+    ids[0] = 1
+    o.set_object(1, temperature=0.5)
+    o.init_fields(rho, u, boundary_id=ids)
+    assert o.thermal
