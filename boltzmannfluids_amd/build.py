@@ -36,25 +36,30 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False, jobs=None):
-    """Compile every object and link libbfamd.so; returns the library path."""
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, jobs=None, out=None, defines=()):
+    """Compile every object and link libbfamd.so; returns the library path.
+    `out` and `defines` (e.g. ["-DBF_NT_STORE=1"]) build a variant elsewhere."""
+    lib = out or LIB
+    build_dir = BUILD if out is None else out + ".objs"
+    extra = list(defines)
+    os.makedirs(build_dir, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     units = []
     for name, vs in VELOCITY_SETS.items():
-        obj = os.path.join(BUILD, f"bf_step_{name}.o")
-        units.append([HIPCC, *COMMON, f"-DBF_VS={vs}", "-c", os.path.join(CSRC, "bf_step.hip"), "-o", obj])
-    units.append([HIPCC, *COMMON, "-c", os.path.join(CSRC, "bf_solver.hip"),
-                  "-o", os.path.join(BUILD, "bf_solver.o")])
+        obj = os.path.join(build_dir, f"bf_step_{name}.o")
+        units.append([HIPCC, *COMMON, *extra, f"-DBF_VS={vs}", "-c", os.path.join(CSRC, "bf_step.hip"), "-o", obj])
+    for src in ("bf_solver", "bf_probe"):
+        units.append([HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src + ".hip"),
+                      "-o", os.path.join(build_dir, src + ".o")])
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for r in ex.map(_run, units):
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
     objs = [u[-1] for u in units]
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -52,11 +52,23 @@ class Info(C.Structure):
 _lib = None
 
 
+def _share_hip_runtime_with_torch():
+    """libbfamd.so and torch both need libamdhip64.so.7; the first copy loaded
+    serves the whole process.  Loading torch first (when it is installed) keeps
+    one HIP runtime, so torch.cuda.synchronize() also covers the solver's
+    stream and torch keeps seeing the GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load_library(path=None):
     """Load libbfamd.so and declare every entry point's signature."""
     global _lib
     if _lib is not None and path is None:
         return _lib
+    _share_hip_runtime_with_torch()
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise FileNotFoundError(
@@ -87,6 +99,7 @@ def load_library(path=None):
         "bf_bits_per_boundary": (i32, [i32]),
         "bf_pack_boundaries": (i32, [C.POINTER(C.c_uint8), i64, i32, C.POINTER(C.c_uint32)]),
         "bf_unpack_boundaries": (i32, [C.POINTER(C.c_uint32), i64, i32, C.POINTER(C.c_uint8)]),
+        "bf_probe_copy_bandwidth": (i32, [i32, i64, i32, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -145,6 +158,16 @@ def unpack_boundaries(words, n, bits):
     if st:
         raise BFError(st, "bf_unpack_boundaries")
     return ids
+
+
+def copy_bandwidth(nbytes=2 << 30, iters=20, device=0):
+    """Sustained device copy bandwidth in GB/s (read + write)."""
+    lib = load_library()
+    out = C.c_double()
+    st = lib.bf_probe_copy_bandwidth(int(device), int(nbytes), int(iters), C.byref(out))
+    if st:
+        raise BFError(st, "bf_probe_copy_bandwidth")
+    return out.value
 
 
 class Solver:

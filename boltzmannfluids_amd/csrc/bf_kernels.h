@@ -63,6 +63,18 @@ struct InitArgs {
     float cs2_t;
 };
 
+// Cells updated per thread by the plain and wall kernels (the multiphase and
+// thermal kernels update one).  Build-time tunable per storage width.
+#ifndef BF_CPT32
+#define BF_CPT32 1
+#endif
+#ifndef BF_CPT16
+#define BF_CPT16 2
+#endif
+constexpr int cells_per_thread(int storage, bool multiphase, bool thermal) {
+    return (multiphase || thermal) ? 1 : (storage == ST_FP32 ? BF_CPT32 : BF_CPT16);
+}
+
 struct LaunchShape {
     dim3 grid, block;
     size_t lds_bytes;
@@ -71,7 +83,7 @@ struct LaunchShape {
 // Block shape for a grid of X x Y x Z cells.  Multiphase kernels stage the
 // pseudopotential of the block tile plus a one-cell halo in LDS and use a
 // 64 x 4 tile; the others use up to 256 consecutive cells of an x-row.
-LaunchShape step_shape(int X, int Y, int Z, bool multiphase, bool three_d);
+LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cells_per_thread);
 
 // Launch one tick.  `partial` (5 doubles per block) receives FP64 block sums
 // of rho, raw momentum and T when non-null.  Returns hipSuccess or the launch error.

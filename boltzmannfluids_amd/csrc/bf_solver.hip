@@ -43,7 +43,7 @@ struct bf_solver {
     float *psi[2] = {nullptr, nullptr};
     float4 *vd = nullptr, *ft = nullptr;
     double *partial = nullptr;
-    size_t partial_blocks = 0;
+    size_t partial_blocks = 0, partial_capacity = 0;
     double *sums_d = nullptr;
     int64_t device_bytes = 0;
 
@@ -329,10 +329,12 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
     if ((st = dalloc(s, &s->objects_d, sizeof(ObjectEntry) * 256)) != BF_OK) return bail(st);
     if ((st = dalloc(s, &s->rho_eff_d, sizeof(float) * 256)) != BF_OK) return bail(st);
     if ((st = dalloc(s, &s->sums_d, sizeof(double) * 8)) != BF_OK) return bail(st);
-    const bool d3 = cfg->velocity_set != BF_D2Q9;
-    s->shape = step_shape(X, Y, Z, cfg->multiphase != 0, d3);
+    // the launch shape depends on whether the flow is thermal, known after
+    // bf_init_fields; size the partial-sum buffer for the larger grid
+    s->shape = step_shape(X, Y, Z, cfg->multiphase != 0, 1);
     s->partial_blocks = (size_t)s->shape.grid.x * s->shape.grid.y * s->shape.grid.z;
-    if ((st = dalloc(s, &s->partial, sizeof(double) * 5 * s->partial_blocks)) != BF_OK) return bail(st);
+    s->partial_capacity = s->partial_blocks;
+    if ((st = dalloc(s, &s->partial, sizeof(double) * 5 * s->partial_capacity)) != BF_OK) return bail(st);
     if (cfg->multiphase) {
         if ((st = dalloc(s, &s->psi[0], sizeof(float) * N)) != BF_OK) return bail(st);
         if ((st = dalloc(s, &s->psi[1], sizeof(float) * N)) != BF_OK) return bail(st);
@@ -444,6 +446,15 @@ bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const f
     }
     bf_status st = upload_objects(s);
     if (st != BF_OK) return st;
+    s->shape = step_shape(s->X, s->Y, s->Z, s->cfg.multiphase != 0,
+                          cells_per_thread(s->storage, s->cfg.multiphase != 0, s->thermal != 0));
+    s->partial_blocks = (size_t)s->shape.grid.x * s->shape.grid.y * s->shape.grid.z;
+    if (s->partial_blocks > s->partial_capacity) {
+        dfree(s->partial);
+        s->partial = nullptr;
+        s->partial_capacity = s->partial_blocks;
+        if ((st = dalloc(s, &s->partial, sizeof(double) * 5 * s->partial_capacity)) != BF_OK) return st;
+    }
 
     // upload rho, u, T and run the equilibrium initialisation
     float *rho_d = nullptr, *u_d = nullptr, *t_d = nullptr;
@@ -726,17 +737,23 @@ bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, u
 
 namespace bf {
 
-LaunchShape step_shape(int X, int Y, int Z, bool multiphase, bool three_d) {
-    (void)three_d;
+#ifndef BF_BLOCK_CELLS
+#define BF_BLOCK_CELLS 256
+#endif
+
+LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cpt) {
     LaunchShape sh{};
     if (multiphase) {
         sh.block = dim3(64, 4, 1);
+        cpt = 1;
     } else {
-        const int bx = std::min(256, (X + 63) / 64 * 64);
-        const int by = std::max(1, 256 / bx);
+        const int per_thread_x = (X + cpt - 1) / cpt;
+        const int bx = std::min(BF_BLOCK_CELLS, (per_thread_x + 63) / 64 * 64);
+        const int by = std::max(1, BF_BLOCK_CELLS / bx);
         sh.block = dim3(bx, by, 1);
     }
-    sh.grid = dim3((X + sh.block.x - 1) / sh.block.x, (Y + sh.block.y - 1) / sh.block.y, Z);
+    const int cells_x = (int)sh.block.x * cpt;
+    sh.grid = dim3((X + cells_x - 1) / cells_x, (Y + sh.block.y - 1) / sh.block.y, Z);
     sh.lds_bytes = 0;
     return sh;
 }

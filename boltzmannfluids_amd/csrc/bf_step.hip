@@ -38,24 +38,50 @@ __device__ inline uint16_t bf16_rne(float v) {
     return (uint16_t)(b >> 16);
 }
 
+// Streaming hints for the population traffic: every slot is read once and
+// written once per tick, so the accesses may bypass cache allocation.
+#ifndef BF_NT_LOAD
+#define BF_NT_LOAD 1
+#endif
+#ifndef BF_NT_STORE
+#define BF_NT_STORE 1
+#endif
+
+template <class T>
+__device__ inline T ld(const T *p) {
+#if BF_NT_LOAD
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <class T>
+__device__ inline void st(T *p, T v) {
+#if BF_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // Storage policies: FP32 as is; FP16 and BF16 hold f - w.
 template <int ST> struct Store;
 template <> struct Store<ST_FP32> {
     using T = float;
-    __device__ static float load(const T *p, int idx, float) { return p[idx]; }
-    __device__ static void store(T *p, int idx, float v, float) { p[idx] = v; }
+    __device__ static float load(const T *p, int idx, float) { return ld(p + idx); }
+    __device__ static void store(T *p, int idx, float v, float) { st(p + idx, v); }
 };
 template <> struct Store<ST_FP16> {
     using T = _Float16;
-    __device__ static float load(const T *p, int idx, float w) { return (float)p[idx] + w; }
-    __device__ static void store(T *p, int idx, float v, float w) { p[idx] = (_Float16)(v - w); }
+    __device__ static float load(const T *p, int idx, float w) { return (float)ld(p + idx) + w; }
+    __device__ static void store(T *p, int idx, float v, float w) { st(p + idx, (_Float16)(v - w)); }
 };
 template <> struct Store<ST_BF16> {
     using T = uint16_t;
     __device__ static float load(const T *p, int idx, float w) {
-        return __uint_as_float((uint32_t)p[idx] << 16) + w;
+        return __uint_as_float((uint32_t)ld(p + idx) << 16) + w;
     }
-    __device__ static void store(T *p, int idx, float v, float w) { p[idx] = bf16_rne(v - w); }
+    __device__ static void store(T *p, int idx, float v, float w) { st(p + idx, bf16_rne(v - w)); }
 };
 
 // c_i . v summed over the non-zero components of c_i in x, y, z order.  Equal
@@ -78,8 +104,20 @@ __device__ inline float dotc(int i, float vx, float vy, float vz) {
     return r;
 }
 
-template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH>
-__global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restrict__ partial) {
+#ifndef BF_WAVES_PER_EU
+#define BF_WAVES_PER_EU 0
+#endif
+#if BF_WAVES_PER_EU
+#define BF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(BF_WAVES_PER_EU)))
+#else
+#define BF_OCCUPANCY
+#endif
+
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT>
+__global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
+    // CPT cells per thread along x, blockDim.x apart: all their loads are issued
+    // before any of them is updated, which keeps more bytes in flight per wave.
+    static_assert(CPT == 1 || (!MP && !TH), "multi-cell threads only for the plain and wall kernels");
     using L = Lattice<VS>;
     using S = Store<ST>;
     using T = typename S::T;
@@ -90,16 +128,10 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restri
     T *__restrict__ f = static_cast<T *>(a.f);
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x = blockIdx.x * blockDim.x + tx;
     const int y = blockIdx.y * blockDim.y + ty;
     const int z = blockIdx.z;
-    const bool inside = x < X && y < Y;
 
     // Neighbour coordinates: index 0 = -1, 1 = 0, 2 = +1 along each axis.
-    int xs[3] = {x - 1, x, x + 1};
-    bool vx[3] = {true, true, true};
-    if (xs[0] < 0) { if (a.periodic_x) xs[0] += X; else vx[0] = false; }
-    if (xs[2] >= X) { if (a.periodic_x) xs[2] -= X; else vx[2] = false; }
     int ys[3] = {y - 1, y, y + 1};
     bool vy[3] = {true, true, true};
     if (ys[0] < 0) { if (a.periodic_y) ys[0] += Y; else vy[0] = false; }
@@ -114,7 +146,21 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restri
     const int ry[3] = {ys[0] * X, ys[1] * X, ys[2] * X};
     const int XY = X * Y;
     const int pz[3] = {zs[0] * XY, zs[1] * XY, zs[2] * XY};
-    const int self = x + ry[1] + pz[1];
+
+    int xs[CPT][3];
+    bool vx[CPT][3];
+    int self[CPT];
+    bool active[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int x = (blockIdx.x * CPT + k) * blockDim.x + tx;
+        xs[k][0] = x - 1; xs[k][1] = x; xs[k][2] = x + 1;
+        vx[k][0] = vx[k][1] = vx[k][2] = true;
+        if (xs[k][0] < 0) { if (a.periodic_x) xs[k][0] += X; else vx[k][0] = false; }
+        if (xs[k][2] >= X) { if (a.periodic_x) xs[k][2] -= X; else vx[k][2] = false; }
+        self[k] = x + ry[1] + pz[1];
+        active[k] = x < X && y < Y;
+    }
 
     // Shan-Chen: stage psi of the previous tick for the block tile + halo.
     __shared__ float tile[MP ? kTileX * kTileY * 3 : 1];
@@ -132,91 +178,102 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restri
         __syncthreads();
     }
 
-    bool active = inside;
-    if (WALLS && active) active = a.ids[self] == 0;  // solid cells skip (collide.comp:696-700)
+    if (WALLS) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+            if (active[k]) active[k] = a.ids[self[k]] == 0;  // solid cells skip (collide.comp:696-700)
+    }
 
     double s_rho = 0.0, s_mx = 0.0, s_my = 0.0, s_mz = 0.0, s_t = 0.0;
 
-    if (active) {
-        // Boundary ids of the Q-1 neighbours x + c_i (even ticks only).
-        uint8_t nb[Q];
-        const bool need_nb = WALLS && !ODD && (a.halfway || TH);
+    // ---- phase 1: boundary ids of the neighbours and AA reads (collide.comp:244-305) ----
+    uint8_t nb[CPT][Q];
+    float fi[CPT][Q];
+    float gi[QT];
+    const bool need_nb = WALLS && !ODD && (a.halfway || TH);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        if (!active[k]) continue;
         if (need_nb) {
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 const int dx = L::c[i][0] + 1, dy = L::c[i][1] + 1, dz = L::c[i][2] + 1;
-                nb[i] = (vx[dx] && vy[dy] && vz[dz]) ? a.ids[xs[dx] + ry[dy] + pz[dz]] : 0;
+                nb[k][i] = (vx[k][dx] && vy[dy] && vz[dz]) ? a.ids[xs[k][dx] + ry[dy] + pz[dz]] : 0;
             }
         }
-
-        // ---- AA read (collide.comp:244-305) ----
-        float fi[Q];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const float w = L::w[i];
             const int o = opposite(i);
             if (ODD || i == 0) {
-                fi[i] = S::load(f + (size_t)(ODD ? o : 0) * a.stride, self, L::w[ODD ? o : 0]);
+                fi[k][i] = S::load(f + (size_t)(ODD ? o : 0) * a.stride, self[k], L::w[ODD ? o : 0]);
             } else {
                 const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
-                if (!(vx[dx] && vy[dy] && vz[dz])) {
-                    fi[i] = 0.01f;  // open boundary read (collide.comp:297-298)
-                } else if (WALLS && a.halfway && nb[o] != 0) {
-                    const ObjectEntry ob = a.objects[nb[o]];
+                if (!(vx[k][dx] && vy[dy] && vz[dz])) {
+                    fi[k][i] = 0.01f;  // open boundary read (collide.comp:297-298)
+                } else if (WALLS && a.halfway && nb[k][o] != 0) {
+                    const ObjectEntry ob = a.objects[nb[k][o]];
                     const float cu = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
-                    fi[i] = S::load(f + (size_t)o * a.stride, self, L::w[o]) + (6.0f * w) * cu;
+                    fi[k][i] = S::load(f + (size_t)o * a.stride, self[k], L::w[o]) + (6.0f * w) * cu;
                 } else {
-                    fi[i] = S::load(f + (size_t)i * a.stride, xs[dx] + ry[dy] + pz[dz], w);
+                    fi[k][i] = S::load(f + (size_t)i * a.stride, xs[k][dx] + ry[dy] + pz[dz], w);
                 }
             }
         }
-
-        // ---- moments (collide.comp:383-396, 704-705) ----
-        float rho = fi[0];
+        if constexpr (TH) {
 #pragma unroll
-        for (int i = 1; i < Q; ++i) rho += fi[i];
+            for (int q = 0; q < QT; ++q) {
+                const int o = opposite(q);
+                if (ODD || q == 0) {
+                    gi[q] = a.g[(size_t)(ODD ? o : 0) * a.stride + self[k]];
+                } else {
+                    const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
+                    if (!(vx[k][dx] && vy[dy] && vz[dz])) {
+                        gi[q] = a.g[(size_t)o * a.stride + self[k]];
+                    } else if (WALLS && nb[k][o] != 0) {
+                        gi[q] = -a.g[(size_t)o * a.stride + self[k]] +
+                                (2.0f * L::wt[q]) * a.objects[nb[k][o]].temperature;
+                    } else {
+                        gi[q] = a.g[(size_t)q * a.stride + xs[k][dx] + ry[dy] + pz[dz]];
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- phase 2: per-cell update and AA writes ----
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        if (!active[k]) continue;
+        // moments (collide.comp:383-396, 704-705)
+        float rho = fi[k][0];
+#pragma unroll
+        for (int i = 1; i < Q; ++i) rho += fi[k][i];
         float mx = 0.0f, my = 0.0f, mz = 0.0f;
         {
             bool fx0 = true, fy0 = true, fz0 = true;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? fi[i] : -fi[i]; mx = fx0 ? t : mx + t; fx0 = false; }
-                if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? fi[i] : -fi[i]; my = fy0 ? t : my + t; fy0 = false; }
-                if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? fi[i] : -fi[i]; mz = fz0 ? t : mz + t; fz0 = false; }
+                const float v = fi[k][i];
+                if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? v : -v; mx = fx0 ? t : mx + t; fx0 = false; }
+                if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? v : -v; my = fy0 ? t : my + t; fy0 = false; }
+                if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? v : -v; mz = fz0 ? t : mz + t; fz0 = false; }
             }
         }
-        if (partial) { s_rho = rho; s_mx = mx; s_my = my; s_mz = mz; }
+        if (partial) { s_rho += rho; s_mx += mx; s_my += my; s_mz += mz; }
         rho = fmaxf(rho, 0.01f);
 
-        // ---- force (collide.comp:707-720) ----
+        // force (collide.comp:707-720)
         float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
         if (a.forcing == 1) { Fx = a.fx; Fy = a.fy; Fz = a.fz; }
-        else if (a.forcing == 2) { const float4 fv = a.force[self]; Fx = fv.x; Fy = fv.y; Fz = fv.z; }
+        else if (a.forcing == 2) { const float4 fv = a.force[self[k]]; Fx = fv.x; Fy = fv.y; Fz = fv.z; }
 
-        float gi[QT];
         float temp = 0.0f;
         if constexpr (TH) {
-#pragma unroll
-            for (int k = 0; k < QT; ++k) {
-                const int o = opposite(k);
-                if (ODD || k == 0) {
-                    gi[k] = a.g[(size_t)(ODD ? o : 0) * a.stride + self];
-                } else {
-                    const int dx = 1 - L::c[k][0], dy = 1 - L::c[k][1], dz = 1 - L::c[k][2];
-                    if (!(vx[dx] && vy[dy] && vz[dz])) {
-                        gi[k] = a.g[(size_t)o * a.stride + self];
-                    } else if (WALLS && nb[o] != 0) {
-                        gi[k] = -a.g[(size_t)o * a.stride + self] +
-                                (2.0f * L::wt[k]) * a.objects[nb[o]].temperature;
-                    } else {
-                        gi[k] = a.g[(size_t)k * a.stride + xs[dx] + ry[dy] + pz[dz]];
-                    }
-                }
-            }
             temp = gi[0];
 #pragma unroll
-            for (int k = 1; k < QT; ++k) temp += gi[k];
-            if (partial) s_t = temp;
+            for (int q = 1; q < QT; ++q) temp += gi[q];
+            if (partial) s_t += temp;
             const float b = -a.beta * (temp - a.t_ref);
             const float bx = b * Fx, by = b * Fy, bz = b * Fz;
             Fx += bx; Fy += by; Fz += bz;
@@ -231,17 +288,17 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restri
             for (int i = 1; i < Q; ++i) {
                 const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                 const int li = (tx + dx) + (ty + dy) * kTileX + (D3 ? dz : 0) * kTileX * kTileY;
-                const float pn = (vx[dx] && vy[dy] && vz[dz]) ? tile[li] : psi_self;
+                const float pn = (vx[k][dx] && vy[dy] && vz[dz]) ? tile[li] : psi_self;
                 const float wp = L::w[i] * pn;
                 if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? wp : -wp; ax = ax0 ? t : ax + t; ax0 = false; }
                 if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? wp : -wp; ay = ay0 ? t : ay + t; ay0 = false; }
                 if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? wp : -wp; az = az0 ? t : az + t; az0 = false; }
             }
-            const float k = -psi_self * a.G;
-            Fx += ax * k; Fy += ay * k; Fz += az * k;
+            const float kk = -psi_self * a.G;
+            Fx += ax * kk; Fy += ay * kk; Fz += az * kk;
         }
 
-        // ---- velocity with half-force shift and clamps (collide.comp:722-725) ----
+        // velocity with half-force shift and clamps (collide.comp:722-725)
         const float r2 = 2.0f * rho;
         float ux = mx / rho + Fx / r2;
         float uy = my / rho + Fy / r2;
@@ -253,7 +310,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restri
             ux *= sc; uy *= sc; uz *= sc;
         }
 
-        // ---- BGK + Guo source, AA write (collide.comp:727-745, 307-369) ----
+        // BGK + Guo source, AA write (collide.comp:727-745, 307-369)
         if (a.update) {
             const float uu = ux * ux + uy * uy + uz * uz;
             const float fu = Fx * ux + Fy * uy + Fz * uz;
@@ -268,65 +325,65 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a, double *__restri
                 if (ext) {
                     v = e;
                 } else {
-                    v = fi[i] - (fi[i] - e) / a.tau;
+                    v = fi[k][i] - (fi[k][i] - e) / a.tau;
                     if (a.source_on) {
                         const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
                         v += pre * w * (3.0f * cf + 9.0f * cf * cu - 3.0f * fu);
                     }
                 }
                 if (ODD || i == 0) {
-                    S::store(f + (size_t)i * a.stride, self, v, w);
+                    S::store(f + (size_t)i * a.stride, self[k], v, w);
                 } else {
                     const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
-                    if (!(vx[dx] && vy[dy] && vz[dz])) {
+                    if (!(vx[k][dx] && vy[dy] && vz[dz])) {
                         // open boundary: write dropped (collide.comp:358-359)
-                    } else if (WALLS && a.halfway && nb[i] != 0) {
-                        const ObjectEntry ob = a.objects[nb[i]];
+                    } else if (WALLS && a.halfway && nb[k][i] != 0) {
+                        const ObjectEntry ob = a.objects[nb[k][i]];
                         const float cw = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
-                        S::store(f + (size_t)i * a.stride, self, v - (6.0f * w) * cw, w);
+                        S::store(f + (size_t)i * a.stride, self[k], v - (6.0f * w) * cw, w);
                     } else {
-                        S::store(f + (size_t)o * a.stride, xs[dx] + ry[dy] + pz[dz], v, w);
+                        S::store(f + (size_t)o * a.stride, xs[k][dx] + ry[dy] + pz[dz], v, w);
                     }
                 }
             }
         }
 
         if constexpr (MP) {
-            if (a.write_psi) a.psi_cur[self] = psi_self;  // collide.comp:763-769 (psi of rho)
+            if (a.write_psi) a.psi_cur[self[k]] = psi_self;  // collide.comp:763-769 (psi of rho)
         }
 
-        // ---- thermal BGK (collide.comp:771-782) ----
+        // thermal BGK (collide.comp:771-782)
         if constexpr (TH) {
             if (a.update) {
                 const float uu = ux * ux + uy * uy + uz * uz;
                 const float cs2 = a.cs2_t;
 #pragma unroll
-                for (int k = 0; k < QT; ++k) {
-                    const int o = opposite(k);
-                    const float cu = (k == 0) ? 0.0f : dotc<BF_VS>(k, ux, uy, uz);
-                    const float e = L::wt[k] * temp *
+                for (int q = 0; q < QT; ++q) {
+                    const int o = opposite(q);
+                    const float cu = (q == 0) ? 0.0f : dotc<BF_VS>(q, ux, uy, uz);
+                    const float e = L::wt[q] * temp *
                                     (1.0f + cu / cs2 + cu * cu / (2.0f * cs2 * cs2) - uu / (2.0f * cs2));
-                    const float v = gi[k] - (gi[k] - e) / a.tau_t;
-                    if (ODD || k == 0) {
-                        a.g[(size_t)k * a.stride + self] = v;
+                    const float v = gi[q] - (gi[q] - e) / a.tau_t;
+                    if (ODD || q == 0) {
+                        a.g[(size_t)q * a.stride + self[k]] = v;
                     } else {
-                        const int dx = 1 + L::c[k][0], dy = 1 + L::c[k][1], dz = 1 + L::c[k][2];
-                        if (!(vx[dx] && vy[dy] && vz[dz])) {
-                            a.g[(size_t)k * a.stride + self] = v;
-                        } else if (WALLS && nb[k] != 0) {
-                            a.g[(size_t)k * a.stride + self] =
-                                -v + (2.0f * L::wt[k]) * a.objects[nb[k]].temperature;
+                        const int dx = 1 + L::c[q][0], dy = 1 + L::c[q][1], dz = 1 + L::c[q][2];
+                        if (!(vx[k][dx] && vy[dy] && vz[dz])) {
+                            a.g[(size_t)q * a.stride + self[k]] = v;
+                        } else if (WALLS && nb[k][q] != 0) {
+                            a.g[(size_t)q * a.stride + self[k]] =
+                                -v + (2.0f * L::wt[q]) * a.objects[nb[k][q]].temperature;
                         } else {
-                            a.g[(size_t)o * a.stride + xs[dx] + ry[dy] + pz[dz]] = v;
+                            a.g[(size_t)o * a.stride + xs[k][dx] + ry[dy] + pz[dz]] = v;
                         }
                     }
                 }
             }
         }
 
-        // ---- macroscopic save (collide.comp:784-792) ----
-        if (a.vd) a.vd[self] = make_float4(ux, uy, uz, rho);
-        if (a.ft) a.ft[self] = make_float4(Fx, Fy, Fz, temp);
+        // macroscopic save (collide.comp:784-792)
+        if (a.vd) a.vd[self[k]] = make_float4(ux, uy, uz, rho);
+        if (a.ft) a.ft[self[k]] = make_float4(Fx, Fy, Fz, temp);
     }
 
     // ---- FP64 block partial sums (deterministic two-stage reduction) ----
@@ -402,10 +459,12 @@ template <int ST, bool ODD>
 hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const LaunchShape &sh,
                           double *partial, hipStream_t stream) {
     constexpr int VS = BF_VS;
-#define BF_L(W, M, T)                                                                              \
-    do {                                                                                           \
-        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T>), sh.grid, sh.block, 0, stream, a, partial); \
-        return hipGetLastError();                                                                  \
+    constexpr int CPT = cells_per_thread(ST, false, false);
+#define BF_L(W, M, T)                                                                                  \
+    do {                                                                                               \
+        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT>), sh.grid, sh.block, 0, stream, a, \
+                           partial);                                                                   \
+        return hipGetLastError();                                                                      \
     } while (0)
     if (walls) {
         if (mp) { if (th) BF_L(true, true, true); else BF_L(true, true, false); }

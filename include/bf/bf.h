@@ -245,6 +245,13 @@ int32_t bf_bits_per_boundary(int32_t object_count);
 bf_status bf_pack_boundaries(const uint8_t *ids, int64_t n, int32_t bits, uint32_t *words);
 bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, uint8_t *ids);
 
+/* --- diagnostics (no counterpart in the reference) ---------------------- */
+
+/* Device copy bandwidth: copies `bytes` between two device buffers `iters`
+ * times and returns (read + write) GB/s.  The achievable-HBM figure beside the
+ * published 8 TB/s, and the known-traffic kernel for counter calibration. */
+bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t iters, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

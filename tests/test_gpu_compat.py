@@ -1,0 +1,39 @@
+"""The C++ drop-in (include/bf/LBM.h) drives the GPU solver like the
+reference's demos drive the GL solver; its result equals the FP32 oracle run
+of the same scenario."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import cavity_ids
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEMO = os.path.join(HERE, "compat", "lid_cavity_demo")
+
+
+def test_lid_cavity_through_lbm_shim(orc, tmp_path):
+    assert os.path.exists(DEMO), "build() compiles tests/compat/lid_cavity_demo"
+    out = tmp_path / "cavity.bin"
+    ticks = 400
+    print(f"case: LBM.h lid cavity 64x64, {ticks} ticks", flush=True)
+    r = subprocess.run([DEMO, str(out), str(ticks)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    t = np.frombuffer(raw[:4], np.int32)[0]
+    n = 64 * 64
+    vd = np.frombuffer(raw[4:4 + 16 * n], np.float32).reshape(n, 4)
+    sums = np.frombuffer(raw[4 + 16 * n:], np.float64)
+    assert t == ticks
+    o = orc.Oracle((64, 64, 1), 0.6, periodic=(0, 0, 1), velocity_set="D2Q9", precision="f32")
+    o.set_object(2, velocity=(np.float32(1.6) / np.float32(16.0), 0, 0))
+    o.init_fields(np.ones(n, np.float32), np.zeros((n, 3), np.float32), boundary_id=cavity_ids(64, 64))
+    o.step(ticks, save=True)
+    ref = o.read("velocity_density")
+    fluid = cavity_ids(64, 64) == 0
+    d = np.abs(vd[fluid].astype(np.float64) - ref[fluid]).max()
+    print(f"  max|shim - oracle| = {d:.3e}", flush=True)
+    assert d <= 1e-6
+    np.testing.assert_allclose(sums[0], o.conservation()[0], rtol=1e-12)
