@@ -49,6 +49,15 @@ class Info(C.Structure):
                 ("cells", C.c_int64), ("stride", C.c_int64), ("device_bytes", C.c_int64)]
 
 
+class ChannelConfig(C.Structure):
+    _fields_ = [("resolution", C.c_int32 * 3), ("velocity_set", C.c_int32), ("relaxation_time", C.c_float),
+                ("inlet_velocity", C.c_float), ("outlet_density", C.c_float), ("device", C.c_int32)]
+
+
+class AdjointConfig(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("checkpoint_every", C.c_int32), ("trajectory_budget_bytes", C.c_int64)]
+
+
 _lib = None
 
 
@@ -100,6 +109,16 @@ def load_library(path=None):
         "bf_pack_boundaries": (i32, [C.POINTER(C.c_uint8), i64, i32, C.POINTER(C.c_uint32)]),
         "bf_unpack_boundaries": (i32, [C.POINTER(C.c_uint32), i64, i32, C.POINTER(C.c_uint8)]),
         "bf_probe_copy_bandwidth": (i32, [i32, i64, i32, C.POINTER(C.c_double)]),
+        "bf_channel_create": (i32, [C.POINTER(ChannelConfig), C.POINTER(vp)]),
+        "bf_channel_destroy": (None, [vp]),
+        "bf_channel_last_error": (C.c_char_p, [vp]),
+        "bf_channel_set_porosity": (i32, [vp, f32p]),
+        "bf_channel_set_inlet_velocity": (i32, [vp, C.c_float]),
+        "bf_channel_forward": (i32, [vp, i64, C.POINTER(C.c_double)]),
+        "bf_channel_gradient": (i32, [vp, C.POINTER(AdjointConfig), C.POINTER(C.c_double), f32p,
+                                      C.POINTER(C.c_double)]),
+        "bf_channel_read_moments": (i32, [vp, f32p, C.c_size_t]),
+        "bf_channel_timing": (i32, [vp, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -296,4 +315,74 @@ class Solver:
     def last_step_sums(self):
         out = np.zeros(5, np.float64)
         self._check(self._lib.bf_last_step_sums(self._h, out.ctypes.data_as(C.POINTER(C.c_double))), "bf_last_step_sums")
+        return out
+
+
+class Channel:
+    """Differentiable channel (bf_channel_*): forward runs and adjoint gradients."""
+
+    def __init__(self, resolution, relaxation_time, inlet_velocity, velocity_set="D2Q9", outlet_density=1.0, device=0):
+        self._lib = load_library()
+        cfg = ChannelConfig()
+        cfg.resolution[:] = [int(r) for r in resolution]
+        cfg.velocity_set = VELOCITY_SETS.get(velocity_set, velocity_set)
+        cfg.relaxation_time = float(relaxation_time)
+        cfg.inlet_velocity = float(inlet_velocity)
+        cfg.outlet_density = float(outlet_density)
+        cfg.device = int(device)
+        h = C.c_void_p()
+        st = self._lib.bf_channel_create(C.byref(cfg), C.byref(h))
+        if st:
+            raise BFError(st, self._lib.bf_channel_last_error(None).decode())
+        self._h = h
+        self.resolution = tuple(int(r) for r in resolution)
+        self.cells = int(np.prod(self.resolution))
+
+    def _check(self, st, what):
+        if st:
+            raise BFError(st, f"{what}: {self._lib.bf_channel_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bf_channel_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_porosity(self, gamma):
+        g = _f32(gamma, self.cells)
+        self._check(self._lib.bf_channel_set_porosity(self._h, _fptr(g)), "bf_channel_set_porosity")
+
+    def set_inlet_velocity(self, u):
+        self._check(self._lib.bf_channel_set_inlet_velocity(self._h, float(u)), "bf_channel_set_inlet_velocity")
+
+    def forward(self, steps):
+        J = C.c_double()
+        self._check(self._lib.bf_channel_forward(self._h, int(steps), C.byref(J)), "bf_channel_forward")
+        return J.value
+
+    def gradient(self, steps, checkpoint_every=100, trajectory_budget_bytes=0):
+        a = AdjointConfig()
+        a.steps = int(steps)
+        a.checkpoint_every = int(checkpoint_every)
+        a.trajectory_budget_bytes = int(trajectory_budget_bytes)
+        J = C.c_double()
+        du = C.c_double()
+        dg = np.zeros(self.cells, np.float32)
+        self._check(self._lib.bf_channel_gradient(self._h, C.byref(a), C.byref(J), _fptr(dg), C.byref(du)),
+                    "bf_channel_gradient")
+        return J.value, dg, du.value
+
+    def moments(self):
+        out = np.zeros(4 * self.cells, np.float32)
+        self._check(self._lib.bf_channel_read_moments(self._h, _fptr(out), out.nbytes), "bf_channel_read_moments")
+        return out.reshape(self.cells, 4)
+
+    def timing(self):
+        out = np.zeros(3, np.float64)
+        self._check(self._lib.bf_channel_timing(self._h, out.ctypes.data_as(C.POINTER(C.c_double))), "bf_channel_timing")
         return out

@@ -245,6 +245,53 @@ int32_t bf_bits_per_boundary(int32_t object_count);
 bf_status bf_pack_boundaries(const uint8_t *ids, int64_t n, int32_t bits, uint32_t *words);
 bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, uint8_t *ids);
 
+/* --- differentiable channel (no counterpart in the reference) ------------
+ *
+ * A separate solver for gradient-based design: a channel with halfway
+ * bounce-back walls at y = 0 and y = Y-1 (z periodic in 3-D), a Zou-He
+ * velocity inlet at x = 0 with a Poiseuille profile of mean `inlet_velocity`,
+ * a Zou-He pressure outlet at x = X-1, and a per-cell porosity gamma in [0, 1]
+ * applied by partial bounce-back.  The objective J is the mean outlet u_x after
+ * `steps` ticks from rest; bf_channel_gradient returns dJ/dgamma (per cell) and
+ * dJ/d inlet_velocity from the hand-written discrete adjoint, recomputing the
+ * forward states between checkpoints taken every `checkpoint_every` ticks. */
+
+typedef struct bf_channel bf_channel;
+
+typedef struct bf_channel_config {
+    int32_t resolution[3];     /* X, Y, Z (Z == 1 for D2Q9) */
+    int32_t velocity_set;      /* BF_D2Q9 or BF_D3Q19 */
+    float relaxation_time;
+    float inlet_velocity;      /* mean u_x of the inlet profile 6 s (1 - s) */
+    float outlet_density;      /* rho imposed at the outlet */
+    int32_t device;
+} bf_channel_config;
+
+typedef struct bf_adjoint_config {
+    int64_t steps;                    /* ticks of the forward run (T) */
+    int32_t checkpoint_every;         /* K: a checkpoint every K ticks */
+    int64_t trajectory_budget_bytes;  /* device bytes for segments kept resident
+                                         instead of recomputed (at least one
+                                         segment buffer is always used) */
+} bf_adjoint_config;
+
+bf_status bf_channel_create(const bf_channel_config *cfg, bf_channel **out);
+void bf_channel_destroy(bf_channel *c);
+const char *bf_channel_last_error(const bf_channel *c);
+/* gamma: N floats in [0, 1], cell order; wall rows are ignored. */
+bf_status bf_channel_set_porosity(bf_channel *c, const float *gamma);
+bf_status bf_channel_set_inlet_velocity(bf_channel *c, float u);
+/* Forward run of `steps` ticks from rest; *J = mean outlet u_x. */
+bf_status bf_channel_forward(bf_channel *c, int64_t steps, double *J);
+/* J, dJ/dgamma (N floats) and dJ/d inlet_velocity by the adjoint. */
+bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *cfg, double *J, float *dJ_dgamma,
+                              double *dJ_duin);
+/* Pre-collision (u_x, u_y, u_z, rho) per cell after the last run (4N floats). */
+bf_status bf_channel_read_moments(bf_channel *c, float *out, size_t bytes);
+/* Device times of the last run in ms: [0] forward (with checkpoints),
+ * [1] recomputation, [2] whole gradient (0 for a forward-only run). */
+bf_status bf_channel_timing(const bf_channel *c, double out[3]);
+
 /* --- diagnostics (no counterpart in the reference) ---------------------- */
 
 /* Device copy bandwidth: copies `bytes` between two device buffers `iters`

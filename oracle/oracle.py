@@ -59,6 +59,16 @@ def load(precision="f64"):
         "orc_unpack": (None, [C.POINTER(C.c_uint32), i64, i32, C.POINTER(C.c_uint8)]),
         "orc_half_rne": (C.c_float, [C.c_float]),
         "orc_bf16_rne": (C.c_float, [C.c_float]),
+        "och_create": (vp, [i32, i32, i32, i32, C.c_double, C.c_double, C.c_double]),
+        "och_destroy": (None, [vp]),
+        "och_set_porosity": (None, [vp, dp]),
+        "och_set_inlet": (None, [vp, C.c_double]),
+        "och_profile": (None, [vp, dp]),
+        "och_forward": (C.c_double, [vp, i64]),
+        "och_moments": (None, [vp, dp]),
+        "och_precollision": (None, [vp, dp, i32, i32, i32, dp]),
+        "och_gradient": (C.c_double, [vp, i64, dp, dp]),
+        "och_tick_pair": (None, [vp, dp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -195,3 +205,71 @@ class Oracle:
         f = _f64(f, self.q * self.cells).copy()
         self._lib.orc_two_lattice(self._h, _dp(f), int(n))
         return f.reshape(self.q, self.cells)
+
+
+class OracleChannel:
+    """CPU restatement of the differentiable channel (bf_oracle_channel.c)."""
+
+    def __init__(self, resolution, relaxation_time, u_in, velocity_set="D2Q9", rho_out=1.0, precision="f64"):
+        self._lib = load(precision)
+        X, Y, Z = (int(r) for r in resolution)
+        vid = VELOCITY_SETS[velocity_set]
+        h = self._lib.och_create(X, Y, Z, vid, float(np.float32(relaxation_time)), float(np.float32(u_in)),
+                                 float(np.float32(rho_out)))
+        if not h:
+            raise ValueError("invalid channel configuration")
+        self._h = h
+        self.resolution = (X, Y, Z)
+        self.cells = X * Y * Z
+        self.q = 9 if vid == 2 else 19
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.och_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_porosity(self, gamma):
+        g = _f64(np.asarray(gamma, np.float32), self.cells)
+        self._lib.och_set_porosity(self._h, _dp(g))
+
+    def set_inlet(self, u_in):
+        self._lib.och_set_inlet(self._h, float(u_in))
+
+    def profile(self):
+        out = np.zeros(self.resolution[1])
+        self._lib.och_profile(self._h, _dp(out))
+        return out
+
+    def forward(self, steps):
+        return float(self._lib.och_forward(self._h, int(steps)))
+
+    def moments(self):
+        out = np.zeros(4 * self.cells)
+        self._lib.och_moments(self._h, _dp(out))
+        return out.reshape(self.cells, 4)
+
+    def precollision(self, state, x, y, z=0):
+        s = _f64(state, self.q * self.cells)
+        out = np.zeros(self.q)
+        self._lib.och_precollision(self._h, _dp(s), int(x), int(y), int(z), _dp(out))
+        return out
+
+    def gradient(self, steps):
+        dg = np.zeros(self.cells)
+        du = np.zeros(1)
+        J = self._lib.och_gradient(self._h, int(steps), _dp(dg), _dp(du))
+        return float(J), dg, float(du[0])
+
+    def tick_pair(self, state, nu):
+        s = _f64(state, self.q * self.cells)
+        n = _f64(nu, self.q * self.cells)
+        out = np.zeros_like(s)
+        nup = np.zeros_like(s)
+        self._lib.och_tick_pair(self._h, _dp(s), _dp(n), _dp(out), _dp(nup))
+        return out, nup

@@ -74,3 +74,17 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     d = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (d if d > 0 else 1.0))
+
+
+def porosity_field(X, Y, Z=1, seed=SEED, scale=1.0):
+    """Porosity gamma in [0, 1]: 3x3(x3) box-smoothed seeded uniform noise,
+    clipped, zero on the wall rows y = 0 and y = Y-1."""
+    rng = np.random.default_rng(seed)
+    a = rng.random((Z, Y, X))
+    k = np.ones(3) / 3.0
+    for ax in (1, 2) if Z == 1 else (0, 1, 2):
+        a = np.apply_along_axis(lambda v: np.convolve(np.pad(v, 1, mode="edge"), k, mode="valid"), ax, a)
+    g = np.clip(scale * a, 0.0, 1.0)
+    g[:, 0, :] = 0.0
+    g[:, -1, :] = 0.0
+    return g.ravel().astype(np.float32)
