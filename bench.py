@@ -97,6 +97,36 @@ def accuracy_vs_fp64(bfa, storage, n=48, ticks=200):
     return {"rho": er, "u": eu, "workload": f"TGV {n}^3 tau=0.55 u0=0.05, {ticks} ticks"}
 
 
+def porosity(X, Y, Z=1, seed=20261203, scale=0.02):
+    """0.02 x (3x3 box-smoothed seeded U[0,1]), zero on the wall rows."""
+    rng = np.random.default_rng(seed)
+    a = rng.random((Z, Y, X))
+    k = np.ones(3) / 3.0
+    for ax in ((1, 2) if Z == 1 else (0, 1, 2)):
+        a = np.apply_along_axis(lambda v: np.convolve(np.pad(v, 1, mode="edge"), k, mode="valid"), ax, a)
+    g = np.clip(scale * a, 0.0, 1.0)
+    g[:, 0, :] = 0.0
+    g[:, -1, :] = 0.0
+    return g.ravel().astype(np.float32)
+
+
+def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
+    """Gradient (forward with checkpoints + recomputation + adjoint) time over
+    the plain forward time of the differentiable channel, device-timed."""
+    ch = bfa.capi.Channel(res, 0.6, 0.05, velocity_set=vs)
+    ch.set_porosity(porosity(*res))
+    ch.forward(steps)                 # warm-up
+    J = ch.forward(steps)
+    fwd = ch.timing()[0]
+    Jg, dg, du = ch.gradient(steps, checkpoint_every=k, trajectory_budget_bytes=budget)
+    t = ch.timing()
+    ch.close()
+    return {"workload": f"channel {vs} {'x'.join(map(str, res))}, tau 0.6, u_in 0.05, {steps} ticks, "
+                        f"checkpoint every {k}, resident budget {budget} B",
+            "J": J, "forward_ms": round(fwd, 3), "gradient_ms": round(t[2], 3), "recompute_ms": round(t[1], 3),
+            "ratio": round(t[2] / fwd, 3), "dJ_duin": du}
+
+
 def cpu_baseline(n=64, ticks=150):
     """FP32 CPU restatement (oracle/liboracle_f32.so), one thread, bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -120,6 +150,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solver runs on a single MI355X; --gpus {args.gpus} is not supported",
@@ -155,6 +186,9 @@ def main():
                                      "roofline": r["roofline"]}
     if not args.no_accuracy:
         line["error_vs_fp64"] = {st: accuracy_vs_fp64(bfa, st) for st in res}
+    if not args.no_adjoint:
+        line["adjoint_over_forward"] = [adjoint_ratio(bfa, (512, 128, 1), "D2Q9"),
+                                        adjoint_ratio(bfa, (128, 128, 128), "D3Q19")]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -69,7 +69,7 @@ struct InitArgs {
 #define BF_CPT32 1
 #endif
 #ifndef BF_CPT16
-#define BF_CPT16 2
+#define BF_CPT16 1
 #endif
 constexpr int cells_per_thread(int storage, bool multiphase, bool thermal) {
     return (multiphase || thermal) ? 1 : (storage == ST_FP32 ? BF_CPT32 : BF_CPT16);
@@ -82,8 +82,13 @@ struct LaunchShape {
 
 // Block shape for a grid of X x Y x Z cells.  Multiphase kernels stage the
 // pseudopotential of the block tile plus a one-cell halo in LDS and use a
-// 64 x 4 tile; the others use up to 256 consecutive cells of an x-row.
-LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cells_per_thread);
+// 64 x 4 tile; the others cover whole x-rows (up to 512 cells per row and
+// thread group) and then get `lds_bytes` of row buffer for the aligned even
+// tick, or row segments of 256 cells (lds_bytes = 0) for longer rows.
+LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cells_per_thread, int velocity_set);
+
+// Directions with c_x != 0 of a velocity set (rows of the even-tick buffer).
+int x_direction_count(int velocity_set);
 
 // Launch one tick.  `partial` (5 doubles per block) receives FP64 block sums
 // of rho, raw momentum and T when non-null.  Returns hipSuccess or the launch error.

@@ -331,7 +331,7 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
     if ((st = dalloc(s, &s->sums_d, sizeof(double) * 8)) != BF_OK) return bail(st);
     // the launch shape depends on whether the flow is thermal, known after
     // bf_init_fields; size the partial-sum buffer for the larger grid
-    s->shape = step_shape(X, Y, Z, cfg->multiphase != 0, 1);
+    s->shape = step_shape(X, Y, Z, cfg->multiphase != 0, 1, cfg->velocity_set);
     s->partial_blocks = (size_t)s->shape.grid.x * s->shape.grid.y * s->shape.grid.z;
     s->partial_capacity = s->partial_blocks;
     if ((st = dalloc(s, &s->partial, sizeof(double) * 5 * s->partial_capacity)) != BF_OK) return bail(st);
@@ -447,7 +447,7 @@ bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const f
     bf_status st = upload_objects(s);
     if (st != BF_OK) return st;
     s->shape = step_shape(s->X, s->Y, s->Z, s->cfg.multiphase != 0,
-                          cells_per_thread(s->storage, s->cfg.multiphase != 0, s->thermal != 0));
+                          cells_per_thread(s->storage, s->cfg.multiphase != 0, s->thermal != 0), s->vs);
     s->partial_blocks = (size_t)s->shape.grid.x * s->shape.grid.y * s->shape.grid.z;
     if (s->partial_blocks > s->partial_capacity) {
         dfree(s->partial);
@@ -741,20 +741,39 @@ namespace bf {
 #define BF_BLOCK_CELLS 256
 #endif
 
-LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cpt) {
+int x_direction_count(int vs) {
+    switch (vs) {
+    case D2Q9: return 6;
+    case D3Q15: return 10;
+    case D3Q19: return 10;
+    case D3Q27: return 18;
+    }
+    return 0;
+}
+
+LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cpt, int vs) {
     LaunchShape sh{};
+    sh.lds_bytes = 0;
     if (multiphase) {
         sh.block = dim3(64, 4, 1);
         cpt = 1;
     } else {
         const int per_thread_x = (X + cpt - 1) / cpt;
-        const int bx = std::min(BF_BLOCK_CELLS, (per_thread_x + 63) / 64 * 64);
-        const int by = std::max(1, BF_BLOCK_CELLS / bx);
-        sh.block = dim3(bx, by, 1);
+        const int row_threads = (per_thread_x + 63) / 64 * 64;
+        if (row_threads <= 512) {
+            // whole rows per block: aligned even tick through an LDS row buffer
+            const int bx = row_threads;
+            const int by = std::max(1, BF_BLOCK_CELLS / bx);
+            sh.block = dim3(bx, by, 1);
+            sh.lds_bytes = sizeof(float) * (size_t)x_direction_count(vs) * by * X;
+        } else {
+            const int bx = std::min(BF_BLOCK_CELLS, row_threads);
+            const int by = std::max(1, BF_BLOCK_CELLS / bx);
+            sh.block = dim3(bx, by, 1);
+        }
     }
     const int cells_x = (int)sh.block.x * cpt;
     sh.grid = dim3((X + cells_x - 1) / cells_x, (Y + sh.block.y - 1) / sh.block.y, Z);
-    sh.lds_bytes = 0;
     return sh;
 }
 

@@ -104,6 +104,23 @@ __device__ inline float dotc(int i, float vx, float vy, float vz) {
     return r;
 }
 
+// Directions with a non-zero x component: their even-tick accesses are shifted
+// by one cell along x and are realigned through LDS (see k_step).
+template <int VS>
+__host__ __device__ constexpr int x_dir_count() {
+    int n = 0;
+    for (int i = 0; i < Lattice<VS>::Q; ++i) n += Lattice<VS>::c[i][0] != 0;
+    return n;
+}
+template <int VS>
+__host__ __device__ constexpr int x_dir_index(int i) {
+    int n = 0;
+    for (int j = 0; j < i; ++j) n += Lattice<VS>::c[j][0] != 0;
+    return n;
+}
+// Marks an LDS entry that no cell wrote (a signalling-NaN pattern that arithmetic never produces).
+constexpr uint32_t kNoValue = 0x7fa5a5a5u;
+
 #ifndef BF_WAVES_PER_EU
 #define BF_WAVES_PER_EU 0
 #endif
@@ -113,11 +130,20 @@ __device__ inline float dotc(int i, float vx, float vy, float vz) {
 #define BF_OCCUPANCY
 #endif
 
-template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT>
-__global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL>
+__global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
     // CPT cells per thread along x, blockDim.x apart: all their loads are issued
-    // before any of them is updated, which keeps more bytes in flight per wave.
+    // before any of them is updated.
+    // AL (even ticks, blocks spanning whole x-rows): the accesses of directions
+    // with c_x != 0 are made aligned.  Each cell loads slot i of the row it pulls
+    // from at its own x into LDS and reads back the entry at x - c_x; on the way
+    // out it writes its value to the LDS entry at x + c_x and the cell owning
+    // that column stores it.  Every global access of the tick then covers whole
+    // cache lines (the periodic x wrap stays inside the block).
     static_assert(CPT == 1 || (!MP && !TH), "multi-cell threads only for the plain and wall kernels");
+    static_assert(!(AL && (ODD || MP)), "aligned rows only for the even tick of non-multiphase kernels");
+    constexpr int NXD = x_dir_count<VS>();
+    extern __shared__ float lds_rows[];
     using L = Lattice<VS>;
     using S = Store<ST>;
     using T = typename S::T;
@@ -191,6 +217,24 @@ __global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, dou
     float fi[CPT][Q];
     float gi[QT];
     const bool need_nb = WALLS && !ODD && (a.halfway || TH);
+    const int lds_row = (int)blockDim.y;  // rows of the block
+    if constexpr (AL) {
+        // every cell of the block's rows stages the aligned slot values it
+        // holds for the cells at x + c_x
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int x = xs[k][1];
+            if (x >= X || y >= Y) continue;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (L::c[i][0] == 0) continue;
+                const int dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
+                float v = 0.0f;
+                if (vy[dy] && vz[dz]) v = S::load(f + (size_t)i * a.stride, x + ry[dy] + pz[dz], L::w[i]);
+                lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + x] = v;
+            }
+        }
+    }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         if (!active[k]) continue;
@@ -215,7 +259,7 @@ __global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, dou
                     const ObjectEntry ob = a.objects[nb[k][o]];
                     const float cu = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
                     fi[k][i] = S::load(f + (size_t)o * a.stride, self[k], L::w[o]) + (6.0f * w) * cu;
-                } else {
+                } else if (!(AL && L::c[i][0] != 0)) {
                     fi[k][i] = S::load(f + (size_t)i * a.stride, xs[k][dx] + ry[dy] + pz[dz], w);
                 }
             }
@@ -237,6 +281,36 @@ __global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, dou
                         gi[q] = a.g[(size_t)q * a.stride + xs[k][dx] + ry[dy] + pz[dz]];
                     }
                 }
+            }
+        }
+    }
+    if constexpr (AL) {
+        __syncthreads();
+        // pull the x-shifted values from the row buffer
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            if (!active[k]) continue;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (L::c[i][0] == 0) continue;
+                const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
+                if (!(vx[k][dx] && vy[dy] && vz[dz])) continue;
+                if (WALLS && a.halfway && nb[k][opposite(i)] != 0) continue;
+                fi[k][i] = lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][dx]];
+            }
+        }
+        __syncthreads();
+        // every entry a cell of the block may write starts out empty
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int x = xs[k][1];
+            if (x >= X || y >= Y) continue;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (L::c[i][0] == 0) continue;
+                const int dx = 1 + L::c[i][0];
+                if (!vx[k][dx]) continue;
+                lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][dx]] = __uint_as_float(kNoValue);
             }
         }
     }
@@ -341,6 +415,8 @@ __global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, dou
                         const ObjectEntry ob = a.objects[nb[k][i]];
                         const float cw = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
                         S::store(f + (size_t)i * a.stride, self[k], v - (6.0f * w) * cw, w);
+                    } else if (AL && L::c[i][0] != 0) {
+                        lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][dx]] = v;
                     } else {
                         S::store(f + (size_t)o * a.stride, xs[k][dx] + ry[dy] + pz[dz], v, w);
                     }
@@ -386,6 +462,26 @@ __global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, dou
         if (a.ft) a.ft[self[k]] = make_float4(Fx, Fy, Fz, temp);
     }
 
+    if constexpr (AL) {
+        __syncthreads();
+        // store the values the row buffer received, each at its own column
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int x = xs[k][1];
+            if (x >= X || y >= Y) continue;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (L::c[i][0] == 0) continue;
+                const int sx = 1 - L::c[i][0];             // the writer sits at x - c_x
+                if (!vx[k][sx]) continue;
+                const float v = lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + x];
+                if (__float_as_uint(v) == kNoValue) continue;
+                const int dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
+                S::store(f + (size_t)opposite(i) * a.stride, x + ry[dy] + pz[dz], v, L::w[i]);
+            }
+        }
+    }
+
     // ---- FP64 block partial sums (deterministic two-stage reduction) ----
     if (partial) {
 #pragma unroll
@@ -396,7 +492,7 @@ __global__ void __launch_bounds__(256) BF_OCCUPANCY k_step(const StepArgs a, dou
             s_mz += __shfl_xor(s_mz, off);
             s_t += __shfl_xor(s_t, off);
         }
-        __shared__ double red[4][5];
+        __shared__ double red[8][5];
         const int tid = ty * blockDim.x + tx;
         const int wave = tid >> 6, lane = tid & 63;
         if (lane == 0) {
@@ -460,11 +556,19 @@ hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const
                           double *partial, hipStream_t stream) {
     constexpr int VS = BF_VS;
     constexpr int CPT = cells_per_thread(ST, false, false);
-#define BF_L(W, M, T)                                                                                  \
-    do {                                                                                               \
-        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT>), sh.grid, sh.block, 0, stream, a, \
-                           partial);                                                                   \
-        return hipGetLastError();                                                                      \
+    const bool al = !ODD && sh.lds_bytes > 0;
+#define BF_L(W, M, T)                                                                                      \
+    do {                                                                                                   \
+        if constexpr (!ODD && !M) {                                                                        \
+            if (al) {                                                                                      \
+                hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, T ? 1 : CPT, true>), sh.grid, sh.block,   \
+                                   sh.lds_bytes, stream, a, partial);                                      \
+                return hipGetLastError();                                                                  \
+            }                                                                                              \
+        }                                                                                                  \
+        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, false>), sh.grid, sh.block, 0,  \
+                           stream, a, partial);                                                            \
+        return hipGetLastError();                                                                          \
     } while (0)
     if (walls) {
         if (mp) { if (th) BF_L(true, true, true); else BF_L(true, true, false); }
