@@ -10,7 +10,10 @@
 //   f**_i = (1 - gamma) f*_i + gamma f_opp(i),   f* = BGK post-collision.
 // The equilibrium and BGK step follow collide.comp:589-629 of the reference.
 //
-// State: s^t = post-collision populations of tick t, population-major.  A tick
+// State: s^t = post-collision populations of tick t, population-major, held
+// shifted (h = f - w, rho = 1 + sum h) so that FP32 sums carry no O(1) part.
+// The closures and partial bounce-back keep their form in h because
+// w_i = w_opp(i) and sum_{c_x=0} w + 2 sum_{c_x=-s} w = 1.  A tick
 // pulls the pre-collision populations of a cell from s^{t-1} (one load per
 // direction), closes the boundaries, collides and writes s^t locally.  The
 // adjoint of a tick runs per cell: it reads dJ/ds^t at the cell, re-derives the
@@ -106,7 +109,7 @@ __device__ inline float closure(const ChArgs &a, float *f, int sign, float ux_in
     }
     ny *= 0.5f;
     nz *= 0.5f;
-    const float rux = sign > 0 ? S * ux_in / (1.0f - ux_in) : S - a.rho_out;
+    const float rux = sign > 0 ? (1.0f + S) * ux_in / (1.0f - ux_in) : S + (1.0f - a.rho_out);
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         if (L::c[i][0] != sign) continue;
@@ -116,7 +119,7 @@ __device__ inline float closure(const ChArgs &a, float *f, int sign, float ux_in
         if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? nz : -nz; tr = first ? t : tr + t; }
         f[i] = f[opposite(i)] + (float)sign * rux * zh_wt<VS>(i) - tr;
     }
-    S_out = S; ny_out = ny; nz_out = nz;
+    S_out = 1.0f + S; ny_out = ny; nz_out = nz;
     return rux;
 }
 
@@ -151,9 +154,10 @@ __global__ void __launch_bounds__(256) k_ch_step(const ChArgs a, const float *__
     int sign;
     float S, ny, nz, uin;
     precollision<VS>(a, src, p, f, sign, S, ny, nz, uin);
-    float rho = f[0];
+    float delta = f[0];
 #pragma unroll
-    for (int i = 1; i < L::Q; ++i) rho += f[i];
+    for (int i = 1; i < L::Q; ++i) delta += f[i];
+    const float rho = 1.0f + delta;
     float mx = 0.0f, my = 0.0f, mz = 0.0f;
     bool bx = true, by = true, bz = true;
 #pragma unroll
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(256) k_ch_step(const ChArgs a, const float *__
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         const float cu = i == 0 ? 0.0f : dotc3<VS>(i, ux, uy, uz);
-        const float e = L::w[i] * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+        const float e = L::w[i] * (delta + rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu));
         fs[i] = f[i] - (f[i] - e) / a.tau;
     }
 #pragma unroll
@@ -189,9 +193,10 @@ __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float 
     int sign;
     float S, ny, nz, uin;
     precollision<VS>(a, src, p, f, sign, S, ny, nz, uin);
-    float rho = f[0];
+    float delta = f[0];
 #pragma unroll
-    for (int i = 1; i < Q; ++i) rho += f[i];
+    for (int i = 1; i < Q; ++i) delta += f[i];
+    const float rho = 1.0f + delta;
     float mx = 0.0f, my = 0.0f, mz = 0.0f;
     bool bx = true, by = true, bz = true;
 #pragma unroll
@@ -212,7 +217,7 @@ __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float 
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const float cu = i == 0 ? 0.0f : dotc3<VS>(i, ux, uy, uz);
-        const float e = L::w[i] * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+        const float e = L::w[i] * (delta + rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu));
         const float fsi = f[i] - (f[i] - e) / a.tau;
         acc += mu[i] * (f[opposite(i)] - fsi);
         const float wm = L::w[i] * mu[i];
@@ -342,7 +347,7 @@ __global__ void __launch_bounds__(256) k_ch_moments(const ChArgs a, const float 
     int sign;
     float S, ny, nz, uin;
     precollision<VS>(a, src, p, f, sign, S, ny, nz, uin);
-    float rho = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+    float rho = 1.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         rho += f[i];
@@ -353,13 +358,14 @@ __global__ void __launch_bounds__(256) k_ch_moments(const ChArgs a, const float 
     out[p.self] = make_float4(mx / rho, my / rho, mz / rho, rho);
 }
 
+// Fluid at rest: f = w, i.e. h = 0.
 template <int VS>
 __global__ void __launch_bounds__(256) k_ch_rest(float *__restrict__ s, int64_t stride, int64_t n) {
     using L = Lattice<VS>;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) s[(size_t)i * stride + c] = L::w[i];
+    for (int i = 0; i < L::Q; ++i) s[(size_t)i * stride + c] = 0.0f;
 }
 
 __global__ void __launch_bounds__(256) k_zero_f32(float *__restrict__ p, int64_t n) {

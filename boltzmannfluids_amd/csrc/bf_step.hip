@@ -64,23 +64,29 @@ __device__ inline void st(T *p, T v) {
 #endif
 }
 
-// Storage policies: FP32 as is; FP16 and BF16 hold f - w.
+// Storage policies: FP32 as is; FP16 and BF16 hold f - w.  `raw` issues the
+// load, `cvt` turns the stored value into f; the tick issues all raw loads of a
+// cell before it converts any, so the loads stay in flight together.
 template <int ST> struct Store;
 template <> struct Store<ST_FP32> {
     using T = float;
-    __device__ static float load(const T *p, int idx, float) { return ld(p + idx); }
+    __device__ static T raw(const T *p, int idx) { return ld(p + idx); }
+    __device__ static float cvt(T v, float) { return v; }
+    __device__ static float load(const T *p, int idx, float w) { return cvt(raw(p, idx), w); }
     __device__ static void store(T *p, int idx, float v, float) { st(p + idx, v); }
 };
 template <> struct Store<ST_FP16> {
     using T = _Float16;
-    __device__ static float load(const T *p, int idx, float w) { return (float)ld(p + idx) + w; }
+    __device__ static T raw(const T *p, int idx) { return ld(p + idx); }
+    __device__ static float cvt(T v, float w) { return (float)v + w; }
+    __device__ static float load(const T *p, int idx, float w) { return cvt(raw(p, idx), w); }
     __device__ static void store(T *p, int idx, float v, float w) { st(p + idx, (_Float16)(v - w)); }
 };
 template <> struct Store<ST_BF16> {
     using T = uint16_t;
-    __device__ static float load(const T *p, int idx, float w) {
-        return __uint_as_float((uint32_t)ld(p + idx) << 16) + w;
-    }
+    __device__ static T raw(const T *p, int idx) { return ld(p + idx); }
+    __device__ static float cvt(T v, float w) { return __uint_as_float((uint32_t)v << 16) + w; }
+    __device__ static float load(const T *p, int idx, float w) { return cvt(raw(p, idx), w); }
     __device__ static void store(T *p, int idx, float v, float w) { st(p + idx, bf16_rne(v - w)); }
 };
 
@@ -213,55 +219,60 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     double s_rho = 0.0, s_mx = 0.0, s_my = 0.0, s_mz = 0.0, s_t = 0.0;
 
     // ---- phase 1: boundary ids of the neighbours and AA reads (collide.comp:244-305) ----
+    // 1a loads the neighbour ids that select addresses, 1b issues every
+    // population load of the tick, 1c converts and applies the open-boundary
+    // and bounce-back rules.  No loaded value is used before 1c.
     uint8_t nb[CPT][Q];
+    T raw[CPT][Q];    // x-directions under AL: the value staged for the row buffer
+    T rawr[CPT][Q];   // x-directions under AL: own opposite slot for a wall redirect
+    float graw[QT];
     float fi[CPT][Q];
     float gi[QT];
     const bool need_nb = WALLS && !ODD && (a.halfway || TH);
     const int lds_row = (int)blockDim.y;  // rows of the block
-    if constexpr (AL) {
-        // every cell of the block's rows stages the aligned slot values it
-        // holds for the cells at x + c_x
-#pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-            const int x = xs[k][1];
-            if (x >= X || y >= Y) continue;
-#pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                if (L::c[i][0] == 0) continue;
-                const int dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
-                float v = 0.0f;
-                if (vy[dy] && vz[dz]) v = S::load(f + (size_t)i * a.stride, x + ry[dy] + pz[dz], L::w[i]);
-                lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + x] = v;
-            }
-        }
-    }
+
+    // 1a
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        if (!active[k]) continue;
-        if (need_nb) {
+        if (!active[k] || !need_nb) continue;
 #pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                const int dx = L::c[i][0] + 1, dy = L::c[i][1] + 1, dz = L::c[i][2] + 1;
-                nb[k][i] = (vx[k][dx] && vy[dy] && vz[dz]) ? a.ids[xs[k][dx] + ry[dy] + pz[dz]] : 0;
+        for (int i = 1; i < Q; ++i) {
+            const int dx = L::c[i][0] + 1, dy = L::c[i][1] + 1, dz = L::c[i][2] + 1;
+            nb[k][i] = (vx[k][dx] && vy[dy] && vz[dz]) ? a.ids[xs[k][dx] + ry[dy] + pz[dz]] : 0;
+        }
+    }
+    // 1b
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        if constexpr (AL) {
+            // every cell of the block's rows loads, at its own x, slot i of the
+            // row the cell at x + c_x pulls from
+            if (xs[k][1] < X && y < Y) {
+#pragma unroll
+                for (int i = 1; i < Q; ++i) {
+                    if (L::c[i][0] == 0) continue;
+                    const int dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
+                    const int idx = (vy[dy] && vz[dz]) ? xs[k][1] + ry[dy] + pz[dz] : self[k];
+                    raw[k][i] = S::raw(f + (size_t)i * a.stride, idx);
+                }
             }
         }
+        if (!active[k]) continue;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const float w = L::w[i];
             const int o = opposite(i);
             if (ODD || i == 0) {
-                fi[k][i] = S::load(f + (size_t)(ODD ? o : 0) * a.stride, self[k], L::w[ODD ? o : 0]);
+                raw[k][i] = S::raw(f + (size_t)(ODD ? o : 0) * a.stride, self[k]);
+                continue;
+            }
+            const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
+            const bool valid = vx[k][dx] && vy[dy] && vz[dz];
+            const bool redir = WALLS && a.halfway && valid && nb[k][o] != 0;
+            if (AL && L::c[i][0] != 0) {
+                if (redir) rawr[k][i] = S::raw(f + (size_t)o * a.stride, self[k]);
             } else {
-                const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
-                if (!(vx[k][dx] && vy[dy] && vz[dz])) {
-                    fi[k][i] = 0.01f;  // open boundary read (collide.comp:297-298)
-                } else if (WALLS && a.halfway && nb[k][o] != 0) {
-                    const ObjectEntry ob = a.objects[nb[k][o]];
-                    const float cu = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
-                    fi[k][i] = S::load(f + (size_t)o * a.stride, self[k], L::w[o]) + (6.0f * w) * cu;
-                } else if (!(AL && L::c[i][0] != 0)) {
-                    fi[k][i] = S::load(f + (size_t)i * a.stride, xs[k][dx] + ry[dy] + pz[dz], w);
-                }
+                const int idx = (valid && !redir) ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
+                raw[k][i] = S::raw(f + (size_t)(redir ? o : i) * a.stride, idx);
             }
         }
         if constexpr (TH) {
@@ -269,18 +280,60 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int q = 0; q < QT; ++q) {
                 const int o = opposite(q);
                 if (ODD || q == 0) {
-                    gi[q] = a.g[(size_t)(ODD ? o : 0) * a.stride + self[k]];
-                } else {
-                    const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
-                    if (!(vx[k][dx] && vy[dy] && vz[dz])) {
-                        gi[q] = a.g[(size_t)o * a.stride + self[k]];
-                    } else if (WALLS && nb[k][o] != 0) {
-                        gi[q] = -a.g[(size_t)o * a.stride + self[k]] +
-                                (2.0f * L::wt[q]) * a.objects[nb[k][o]].temperature;
-                    } else {
-                        gi[q] = a.g[(size_t)q * a.stride + xs[k][dx] + ry[dy] + pz[dz]];
-                    }
+                    graw[q] = a.g[(size_t)(ODD ? o : 0) * a.stride + self[k]];
+                    continue;
                 }
+                const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
+                const bool pull = vx[k][dx] && vy[dy] && vz[dz] && !(WALLS && nb[k][o] != 0);
+                graw[q] = a.g[(size_t)(pull ? q : o) * a.stride + (pull ? xs[k][dx] + ry[dy] + pz[dz] : self[k])];
+            }
+        }
+    }
+    // 1c
+    if constexpr (AL) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            if (!(xs[k][1] < X && y < Y)) continue;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (L::c[i][0] == 0) continue;
+                lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][1]] = S::cvt(raw[k][i], L::w[i]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        if (!active[k]) continue;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const float w = L::w[i];
+            const int o = opposite(i);
+            if (ODD || i == 0) {
+                fi[k][i] = S::cvt(raw[k][i], L::w[ODD ? o : 0]);
+                continue;
+            }
+            const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
+            const bool valid = vx[k][dx] && vy[dy] && vz[dz];
+            const bool redir = WALLS && a.halfway && valid && nb[k][o] != 0;
+            if (!valid) {
+                fi[k][i] = 0.01f;  // open boundary read (collide.comp:297-298)
+            } else if (redir) {
+                const ObjectEntry ob = a.objects[nb[k][o]];
+                const float cu = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
+                fi[k][i] = S::cvt((AL && L::c[i][0] != 0) ? rawr[k][i] : raw[k][i], L::w[o]) + (6.0f * w) * cu;
+            } else if (!(AL && L::c[i][0] != 0)) {
+                fi[k][i] = S::cvt(raw[k][i], w);
+            }
+        }
+        if constexpr (TH) {
+#pragma unroll
+            for (int q = 0; q < QT; ++q) {
+                gi[q] = graw[q];
+                if (ODD || q == 0) continue;
+                const int o = opposite(q);
+                const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
+                if (vx[k][dx] && vy[dy] && vz[dz] && WALLS && nb[k][o] != 0)
+                    gi[q] = -graw[q] + (2.0f * L::wt[q]) * a.objects[nb[k][o]].temperature;
             }
         }
     }

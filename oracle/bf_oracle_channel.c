@@ -22,7 +22,11 @@
  *
  * State convention: s^t = post-collision populations after tick t (stored per
  * cell, population-major); the pre-collision populations of tick t are pulled
- * from s^{t-1}; s^{-1} is the fluid at rest (f = w).
+ * from s^{t-1}; s^{-1} is the fluid at rest (f = w).  All populations are held
+ * shifted, h = f - w, and rho = 1 + sum h, so FP32 sums carry no O(1) part
+ * (the same FP32 accuracy device as the shifted 16-bit storage of the solver).
+ * Every closure and the partial bounce-back keep their form in h because
+ * w_i = w_opp(i) and sum_{c_x=0} w + 2 sum_{c_x=-s} w = 1.
  *
  * Parity pins (tests/test_oracle_channel.py): Zou-He reconstruction reproduces
  * the imposed moments; the adjoint gradient equals central finite differences
@@ -92,8 +96,9 @@ static void pull(const och *s, const REAL *src, int x, int y, int z, REAL *f) {
 }
 
 /* Boundary closure at the inlet (x = 0, sign +1) or outlet (x = X-1, sign -1).
- * Returns rho u_x; fills the unknown populations in f.  Also returns the sum
- * S of the known populations (c_x = 0 once, c_x = -sign twice). */
+ * Returns rho u_x; fills the unknown populations in f (shifted values).  S_out
+ * receives the full sum S = 1 + S_h of the known populations (c_x = 0 once,
+ * c_x = -sign twice). */
 static REAL closure(const och *s, REAL *f, int sign, REAL ux_in, REAL *S_out, REAL *Ny, REAL *Nz) {
     REAL S = 0, ny = 0, nz = 0;
     for (int i = 0; i < s->q; ++i) {
@@ -103,32 +108,34 @@ static REAL closure(const och *s, REAL *f, int sign, REAL ux_in, REAL *S_out, RE
     }
     ny *= (REAL)0.5; nz *= (REAL)0.5;
     REAL rux;
-    if (sign > 0) rux = S * ux_in / ((REAL)1 - ux_in);   /* rho = S/(1-u) */
-    else rux = S - s->rho_out;                            /* rho u = rho_out (S/rho_out - 1) */
+    if (sign > 0) rux = ((REAL)1 + S) * ux_in / ((REAL)1 - ux_in);   /* rho = S/(1-u) */
+    else rux = S + ((REAL)1 - s->rho_out);                            /* rho u = rho_out (S/rho_out - 1) */
     for (int i = 0; i < s->q; ++i) {
         const int *c = s->T[i].c;
         if (c[0] != sign) continue;
         f[i] = f[opp(i)] + (REAL)sign * rux * zh_wt(s, i) - ((REAL)c[1] * ny + (REAL)c[2] * nz);
     }
-    *S_out = S; *Ny = ny; *Nz = nz;
+    *S_out = (REAL)1 + S; *Ny = ny; *Nz = nz;
     return rux;
 }
 
 /* Collision + partial bounce-back of one cell; f (pre) -> out (post). Also
  * returns u and rho, and f* (BGK post-collision) in fstar when non-null. */
 static void collide(const och *s, const REAL *f, REAL g, REAL *out, REAL *fstar, REAL *rho_o, REAL *u_o) {
-    REAL rho = 0, m[3] = {0, 0, 0};
+    REAL delta = 0, m[3] = {0, 0, 0};
     for (int i = 0; i < s->q; ++i) {
-        rho += f[i];
+        delta += f[i];
         m[0] += f[i] * s->T[i].c[0]; m[1] += f[i] * s->T[i].c[1]; m[2] += f[i] * s->T[i].c[2];
     }
+    REAL rho = (REAL)1 + delta;
     REAL u[3] = {m[0] / rho, m[1] / rho, m[2] / rho};
     REAL uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     REAL fs[19];
     for (int i = 0; i < s->q; ++i) {
         const int *c = s->T[i].c;
         REAL cu = c[0] * u[0] + c[1] * u[1] + c[2] * u[2];
-        REAL e = (REAL)s->T[i].w * rho * ((REAL)1 + (REAL)3 * cu + (REAL)4.5 * cu * cu - (REAL)1.5 * uu);
+        /* f_eq - w = w (rho - 1) + w rho (3 cu + 9/2 cu^2 - 3/2 u^2) */
+        REAL e = (REAL)s->T[i].w * (delta + rho * ((REAL)3 * cu + (REAL)4.5 * cu * cu - (REAL)1.5 * uu));
         fs[i] = f[i] - (f[i] - e) / s->tau;
     }
     for (int i = 0; i < s->q; ++i) out[i] = ((REAL)1 - g) * fs[i] + g * f[opp(i)];
@@ -161,8 +168,7 @@ static void tick(och *s, const REAL *src, REAL *dst) {
 }
 
 static void rest_state(const och *s, REAL *dst) {
-    for (int i = 0; i < s->q; ++i)
-        for (long c = 0; c < s->N; ++c) dst[(long)i * s->N + c] = (REAL)s->T[i].w;
+    memset(dst, 0, sizeof(REAL) * s->q * s->N);   /* f = w */
 }
 
 /* J = mean over outlet fluid cells of u_x of the pre-collision populations
@@ -236,7 +242,7 @@ OCH_API void och_moments(const och *s, double *out) {
     for (int y = 1; y < s->Y - 1; ++y)
     for (int x = 0; x < s->X; ++x) {
         precollision(s, s->state[s->cur], x, y, z, f);
-        REAL rho = 0, m[3] = {0, 0, 0};
+        REAL rho = 1, m[3] = {0, 0, 0};
         for (int i = 0; i < s->q; ++i) {
             rho += f[i];
             m[0] += f[i] * s->T[i].c[0]; m[1] += f[i] * s->T[i].c[1]; m[2] += f[i] * s->T[i].c[2];
@@ -246,14 +252,14 @@ OCH_API void och_moments(const och *s, double *out) {
     }
 }
 
-/* Pre-collision populations of a cell after closure, for the closure tests:
- * `src` is an arbitrary post-collision state (q*N). */
+/* Pre-collision populations (unshifted) of a cell after closure, for the
+ * closure tests: `src` is an arbitrary post-collision state (q*N, unshifted). */
 OCH_API void och_precollision(const och *s, const double *src, int x, int y, int z, double *f) {
     REAL *tmp = (REAL *)malloc(sizeof(REAL) * s->q * s->N);
-    for (long k = 0; k < (long)s->q * s->N; ++k) tmp[k] = (REAL)src[k];
+    for (long k = 0; k < (long)s->q * s->N; ++k) tmp[k] = (REAL)(src[k] - s->T[k / s->N].w);
     REAL ff[19];
     precollision(s, tmp, x, y, z, ff);
-    for (int i = 0; i < s->q; ++i) f[i] = (double)ff[i];
+    for (int i = 0; i < s->q; ++i) f[i] = (double)ff[i] + s->T[i].w;
     free(tmp);
 }
 
@@ -376,8 +382,8 @@ OCH_API double och_gradient(och *s, long T, double *dgamma, double *du_out) {
     return J;
 }
 
-/* One tick and its adjoint on arbitrary inputs, for the dot-product test:
- * out = tick(src); nu_prev = (d tick / d src)^T nu. */
+/* One tick and its adjoint on arbitrary inputs (shifted values), for the
+ * dot-product test: out = tick(src); nu_prev = (d tick / d src)^T nu. */
 OCH_API void och_tick_pair(och *s, const double *src, const double *nu, double *out, double *nu_prev) {
     long qN = (long)s->q * s->N;
     REAL *a = (REAL *)calloc(qN, sizeof(REAL)), *b = (REAL *)calloc(qN, sizeof(REAL));
