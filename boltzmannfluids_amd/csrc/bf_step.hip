@@ -70,24 +70,24 @@ __device__ inline void st(T *p, T v) {
 template <int ST> struct Store;
 template <> struct Store<ST_FP32> {
     using T = float;
-    __device__ static T raw(const T *p, int idx) { return ld(p + idx); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld(p + idx); }
     __device__ static float cvt(T v, float) { return v; }
-    __device__ static float load(const T *p, int idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, int idx, float v, float) { st(p + idx, v); }
+    __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
+    __device__ static void store(T *p, uint32_t idx, float v, float) { st(p + idx, v); }
 };
 template <> struct Store<ST_FP16> {
     using T = _Float16;
-    __device__ static T raw(const T *p, int idx) { return ld(p + idx); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld(p + idx); }
     __device__ static float cvt(T v, float w) { return (float)v + w; }
-    __device__ static float load(const T *p, int idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, int idx, float v, float w) { st(p + idx, (_Float16)(v - w)); }
+    __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
+    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(p + idx, (_Float16)(v - w)); }
 };
 template <> struct Store<ST_BF16> {
     using T = uint16_t;
-    __device__ static T raw(const T *p, int idx) { return ld(p + idx); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld(p + idx); }
     __device__ static float cvt(T v, float w) { return __uint_as_float((uint32_t)v << 16) + w; }
-    __device__ static float load(const T *p, int idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, int idx, float v, float w) { st(p + idx, bf16_rne(v - w)); }
+    __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
+    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(p + idx, bf16_rne(v - w)); }
 };
 
 // c_i . v summed over the non-zero components of c_i in x, y, z order.  Equal
@@ -136,8 +136,10 @@ constexpr uint32_t kNoValue = 0x7fa5a5a5u;
 #define BF_OCCUPANCY
 #endif
 
-template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL>
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER>
 __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
+    // PER: every axis periodic (and no walls): every neighbour exists, so the
+    // open-boundary branches vanish at compile time.
     // CPT cells per thread along x, blockDim.x apart: all their loads are issued
     // before any of them is updated.
     // AL (even ticks, blocks spanning whole x-rows): the accesses of directions
@@ -148,6 +150,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     // cache lines (the periodic x wrap stays inside the block).
     static_assert(CPT == 1 || (!MP && !TH), "multi-cell threads only for the plain and wall kernels");
     static_assert(!(AL && (ODD || MP)), "aligned rows only for the even tick of non-multiphase kernels");
+    static_assert(!(PER && (WALLS || MP || TH)), "the periodic specialisation is the plain kernel");
     constexpr int NXD = x_dir_count<VS>();
     extern __shared__ float lds_rows[];
     using L = Lattice<VS>;
@@ -179,6 +182,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     const int XY = X * Y;
     const int pz[3] = {zs[0] * XY, zs[1] * XY, zs[2] * XY};
 
+    // neighbour (dx, dy, dz) of cell k inside the domain (indices 0/1/2 = -1/0/+1)
+#define OK(k, dx, dy, dz) (PER || (vx[k][dx] && vy[dy] && vz[dz]))
     int xs[CPT][3];
     bool vx[CPT][3];
     int self[CPT];
@@ -229,7 +234,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     float fi[CPT][Q];
     float gi[QT];
     const bool need_nb = WALLS && !ODD && (a.halfway || TH);
-    const int lds_row = (int)blockDim.y;  // rows of the block
+    const int lds_plane = (int)blockDim.y * X;  // row buffer: [x-direction][row of the block][x]
+    const int lds_base = ty * X;
+#define LDS(i, col) lds_rows[x_dir_index<VS>(i) * lds_plane + lds_base + (col)]
 
     // 1a
 #pragma unroll
@@ -238,7 +245,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #pragma unroll
         for (int i = 1; i < Q; ++i) {
             const int dx = L::c[i][0] + 1, dy = L::c[i][1] + 1, dz = L::c[i][2] + 1;
-            nb[k][i] = (vx[k][dx] && vy[dy] && vz[dz]) ? a.ids[xs[k][dx] + ry[dy] + pz[dz]] : 0;
+            nb[k][i] = OK(k, dx, dy, dz) ? a.ids[xs[k][dx] + ry[dy] + pz[dz]] : 0;
         }
     }
     // 1b
@@ -252,7 +259,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 for (int i = 1; i < Q; ++i) {
                     if (L::c[i][0] == 0) continue;
                     const int dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
-                    const int idx = (vy[dy] && vz[dz]) ? xs[k][1] + ry[dy] + pz[dz] : self[k];
+                    const int idx = (PER || (vy[dy] && vz[dz])) ? xs[k][1] + ry[dy] + pz[dz] : self[k];
                     raw[k][i] = S::raw(f + (size_t)i * a.stride, idx);
                 }
             }
@@ -284,7 +291,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     continue;
                 }
                 const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
-                const bool pull = vx[k][dx] && vy[dy] && vz[dz] && !(WALLS && nb[k][o] != 0);
+                const bool pull = OK(k, dx, dy, dz) && !(WALLS && nb[k][o] != 0);
                 graw[q] = a.g[(size_t)(pull ? q : o) * a.stride + (pull ? xs[k][dx] + ry[dy] + pz[dz] : self[k])];
             }
         }
@@ -297,7 +304,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
-                lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][1]] = S::cvt(raw[k][i], L::w[i]);
+                LDS(i, xs[k][1]) = S::cvt(raw[k][i], L::w[i]);
             }
         }
     }
@@ -320,7 +327,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             } else if (redir) {
                 const ObjectEntry ob = a.objects[nb[k][o]];
                 const float cu = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
-                fi[k][i] = S::cvt((AL && L::c[i][0] != 0) ? rawr[k][i] : raw[k][i], L::w[o]) + (6.0f * w) * cu;
+                fi[k][i] = __builtin_fmaf(6.0f * w, cu, S::cvt((AL && L::c[i][0] != 0) ? rawr[k][i] : raw[k][i], L::w[o]));
             } else if (!(AL && L::c[i][0] != 0)) {
                 fi[k][i] = S::cvt(raw[k][i], w);
             }
@@ -332,8 +339,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 if (ODD || q == 0) continue;
                 const int o = opposite(q);
                 const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
-                if (vx[k][dx] && vy[dy] && vz[dz] && WALLS && nb[k][o] != 0)
-                    gi[q] = -graw[q] + (2.0f * L::wt[q]) * a.objects[nb[k][o]].temperature;
+                if (OK(k, dx, dy, dz) && WALLS && nb[k][o] != 0)
+                    gi[q] = __builtin_fmaf(2.0f * L::wt[q], a.objects[nb[k][o]].temperature, -graw[q]);
             }
         }
     }
@@ -347,9 +354,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
                 const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
-                if (!(vx[k][dx] && vy[dy] && vz[dz])) continue;
+                if (!OK(k, dx, dy, dz)) continue;
                 if (WALLS && a.halfway && nb[k][opposite(i)] != 0) continue;
-                fi[k][i] = lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][dx]];
+                fi[k][i] = LDS(i, xs[k][dx]);
             }
         }
         __syncthreads();
@@ -362,8 +369,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
                 const int dx = 1 + L::c[i][0];
-                if (!vx[k][dx]) continue;
-                lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][dx]] = __uint_as_float(kNoValue);
+                if (!(PER || vx[k][dx])) continue;
+                LDS(i, xs[k][dx]) = __uint_as_float(kNoValue);
             }
         }
     }
@@ -415,7 +422,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int i = 1; i < Q; ++i) {
                 const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                 const int li = (tx + dx) + (ty + dy) * kTileX + (D3 ? dz : 0) * kTileX * kTileY;
-                const float pn = (vx[k][dx] && vy[dy] && vz[dz]) ? tile[li] : psi_self;
+                const float pn = OK(k, dx, dy, dz) ? tile[li] : psi_self;
                 const float wp = L::w[i] * pn;
                 if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? wp : -wp; ax = ax0 ? t : ax + t; ax0 = false; }
                 if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? wp : -wp; ay = ay0 ? t : ay + t; ay0 = false; }
@@ -426,11 +433,11 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         }
 
         // velocity with half-force shift and clamps (collide.comp:722-725)
-        const float r2 = 2.0f * rho;
-        float ux = mx / rho + Fx / r2;
-        float uy = my / rho + Fy / r2;
-        float uz = mz / rho + Fz / r2;
-        const float un = sqrtf(ux * ux + uy * uy + uz * uz);
+        const float inv_rho = 1.0f / rho, half_inv = 0.5f * inv_rho;
+        float ux = __builtin_fmaf(Fx, half_inv, mx * inv_rho);
+        float uy = __builtin_fmaf(Fy, half_inv, my * inv_rho);
+        float uz = __builtin_fmaf(Fz, half_inv, mz * inv_rho);
+        const float un = sqrtf(__builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux)));
         const bool ext = un > a.uext;
         if (un > a.ulim) {
             const float sc = a.ulim / un;
@@ -439,37 +446,40 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 
         // BGK + Guo source, AA write (collide.comp:727-745, 307-369)
         if (a.update) {
-            const float uu = ux * ux + uy * uy + uz * uz;
-            const float fu = Fx * ux + Fy * uy + Fz * uz;
+            const float uu = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
+            const float fu = __builtin_fmaf(Fz, uz, __builtin_fmaf(Fy, uy, Fx * ux));
+            const float c1 = __builtin_fmaf(-1.5f, uu, 1.0f);
+            const float inv_tau = 1.0f / a.tau;
             const float pre = 1.0f - 1.0f / (2.0f * a.tau);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 const float w = L::w[i];
                 const int o = opposite(i);
                 const float cu = (i == 0) ? 0.0f : dotc<BF_VS>(i, ux, uy, uz);
-                const float e = w * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+                // w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2)
+                const float e = (w * rho) * __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1);
                 float v;
                 if (ext) {
                     v = e;
                 } else {
-                    v = fi[k][i] - (fi[k][i] - e) / a.tau;
+                    v = __builtin_fmaf(e - fi[k][i], inv_tau, fi[k][i]);   // f - (f - feq)/tau
                     if (a.source_on) {
                         const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
-                        v += pre * w * (3.0f * cf + 9.0f * cf * cu - 3.0f * fu);
+                        v += (pre * w) * (__builtin_fmaf(9.0f * cf, cu, 3.0f * cf) - 3.0f * fu);
                     }
                 }
                 if (ODD || i == 0) {
                     S::store(f + (size_t)i * a.stride, self[k], v, w);
                 } else {
                     const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
-                    if (!(vx[k][dx] && vy[dy] && vz[dz])) {
+                    if (!OK(k, dx, dy, dz)) {
                         // open boundary: write dropped (collide.comp:358-359)
                     } else if (WALLS && a.halfway && nb[k][i] != 0) {
                         const ObjectEntry ob = a.objects[nb[k][i]];
                         const float cw = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
-                        S::store(f + (size_t)i * a.stride, self[k], v - (6.0f * w) * cw, w);
+                        S::store(f + (size_t)i * a.stride, self[k], __builtin_fmaf(-6.0f * w, cw, v), w);
                     } else if (AL && L::c[i][0] != 0) {
-                        lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + xs[k][dx]] = v;
+                        LDS(i, xs[k][dx]) = v;
                     } else {
                         S::store(f + (size_t)o * a.stride, xs[k][dx] + ry[dy] + pz[dz], v, w);
                     }
@@ -484,24 +494,26 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         // thermal BGK (collide.comp:771-782)
         if constexpr (TH) {
             if (a.update) {
-                const float uu = ux * ux + uy * uy + uz * uz;
-                const float cs2 = a.cs2_t;
+                const float uu = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
+                const float k1 = 1.0f / a.cs2_t, k2 = 0.5f * k1 * k1;
+                const float t1 = __builtin_fmaf(-0.5f * k1, uu, 1.0f);
+                const float inv_tau_t = 1.0f / a.tau_t;
 #pragma unroll
                 for (int q = 0; q < QT; ++q) {
                     const int o = opposite(q);
                     const float cu = (q == 0) ? 0.0f : dotc<BF_VS>(q, ux, uy, uz);
-                    const float e = L::wt[q] * temp *
-                                    (1.0f + cu / cs2 + cu * cu / (2.0f * cs2 * cs2) - uu / (2.0f * cs2));
-                    const float v = gi[q] - (gi[q] - e) / a.tau_t;
+                    // wt T (1 + cu/cs2 + cu^2/(2 cs2^2) - u^2/(2 cs2))
+                    const float e = (L::wt[q] * temp) * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
+                    const float v = __builtin_fmaf(e - gi[q], inv_tau_t, gi[q]);
                     if (ODD || q == 0) {
                         a.g[(size_t)q * a.stride + self[k]] = v;
                     } else {
                         const int dx = 1 + L::c[q][0], dy = 1 + L::c[q][1], dz = 1 + L::c[q][2];
-                        if (!(vx[k][dx] && vy[dy] && vz[dz])) {
+                        if (!OK(k, dx, dy, dz)) {
                             a.g[(size_t)q * a.stride + self[k]] = v;
                         } else if (WALLS && nb[k][q] != 0) {
                             a.g[(size_t)q * a.stride + self[k]] =
-                                -v + (2.0f * L::wt[q]) * a.objects[nb[k][q]].temperature;
+                                __builtin_fmaf(2.0f * L::wt[q], a.objects[nb[k][q]].temperature, -v);
                         } else {
                             a.g[(size_t)o * a.stride + xs[k][dx] + ry[dy] + pz[dz]] = v;
                         }
@@ -526,8 +538,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
                 const int sx = 1 - L::c[i][0];             // the writer sits at x - c_x
-                if (!vx[k][sx]) continue;
-                const float v = lds_rows[(x_dir_index<VS>(i) * lds_row + ty) * X + x];
+                if (!(PER || vx[k][sx])) continue;
+                const float v = LDS(i, x);
                 if (__float_as_uint(v) == kNoValue) continue;
                 const int dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                 S::store(f + (size_t)opposite(i) * a.stride, x + ry[dy] + pz[dz], v, L::w[i]);
@@ -535,6 +547,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         }
     }
 
+#undef LDS
+#undef OK
     // ---- FP64 block partial sums (deterministic two-stage reduction) ----
     if (partial) {
 #pragma unroll
@@ -579,21 +593,22 @@ __global__ void __launch_bounds__(256) k_init(const InitArgs a) {
     float ux = a.u[3 * c], uy = a.u[3 * c + 1], uz = a.u[3 * c + 2];
     const uint8_t id = a.ids ? a.ids[c] : 0;
     if (id) { ux = 0.0f; uy = 0.0f; uz = 0.0f; }
-    const float uu = ux * ux + uy * uy + uz * uz;
+    const float uu = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
+    const float c1 = __builtin_fmaf(-1.5f, uu, 1.0f);
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
-        const float cu = (float)L::c[i][0] * ux + (float)L::c[i][1] * uy + (float)L::c[i][2] * uz;
-        const float e = L::w[i] * r * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
-        S::store(f + (size_t)i * a.stride, (int)c, e, L::w[i]);
+        const float cu = i == 0 ? 0.0f : dotc<VS>(i, ux, uy, uz);
+        const float e = (L::w[i] * r) * __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1);
+        S::store(f + (size_t)i * a.stride, (uint32_t)c, e, L::w[i]);
     }
     if constexpr (TH) {
         const float T0 = a.temperature ? a.temperature[c] : 1.0f;
-        const float cs2 = a.cs2_t;
+        const float k1 = 1.0f / a.cs2_t, k2 = 0.5f * k1 * k1;
+        const float t1 = __builtin_fmaf(-0.5f * k1, uu, 1.0f);
 #pragma unroll
         for (int k = 0; k < L::QT; ++k) {
-            const float cu = (float)L::c[k][0] * ux + (float)L::c[k][1] * uy + (float)L::c[k][2] * uz;
-            const float e = L::wt[k] * T0 * (1.0f + cu / cs2 + cu * cu / (2.0f * cs2 * cs2) - uu / (2.0f * cs2));
-            a.g[(size_t)k * a.stride + c] = e;
+            const float cu = k == 0 ? 0.0f : dotc<VS>(k, ux, uy, uz);
+            a.g[(size_t)k * a.stride + c] = (L::wt[k] * T0) * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
         }
     }
     if (a.psi0) {
@@ -610,16 +625,28 @@ hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const
     constexpr int VS = BF_VS;
     constexpr int CPT = cells_per_thread(ST, false, false);
     const bool al = !ODD && sh.lds_bytes > 0;
+    if (!walls && !mp && !th && a.periodic_x && a.periodic_y && a.periodic_z) {
+        if constexpr (!ODD) {
+            if (al) {
+                hipLaunchKernelGGL((k_step<VS, ST, ODD, false, false, false, CPT, true, true>), sh.grid, sh.block,
+                                   sh.lds_bytes, stream, a, partial);
+                return hipGetLastError();
+            }
+        }
+        hipLaunchKernelGGL((k_step<VS, ST, ODD, false, false, false, CPT, false, true>), sh.grid, sh.block, 0, stream,
+                           a, partial);
+        return hipGetLastError();
+    }
 #define BF_L(W, M, T)                                                                                      \
     do {                                                                                                   \
         if constexpr (!ODD && !M) {                                                                        \
             if (al) {                                                                                      \
-                hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, T ? 1 : CPT, true>), sh.grid, sh.block,   \
+                hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, T ? 1 : CPT, true, false>), sh.grid, sh.block, \
                                    sh.lds_bytes, stream, a, partial);                                      \
                 return hipGetLastError();                                                                  \
             }                                                                                              \
         }                                                                                                  \
-        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, false>), sh.grid, sh.block, 0,  \
+        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, false, false>), sh.grid, sh.block, 0, \
                            stream, a, partial);                                                            \
         return hipGetLastError();                                                                          \
     } while (0)

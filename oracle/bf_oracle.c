@@ -49,6 +49,14 @@
 
 #define ORC_API __attribute__((visibility("default")))
 
+/* Fused multiply-add of REAL, placed at the same spots as in the HIP kernel
+ * (hardware FMA on both sides, one rounding). */
+#ifdef ORACLE_F32
+#define FMA(a, b, c) fmaf((a), (b), (c))
+#else
+#define FMA(a, b, c) fma((a), (b), (c))
+#endif
+
 /* ---------------------------------------------------------------------- */
 /* Direction tables (VelocitySet.cpp:53-141, 180-203)                      */
 
@@ -247,14 +255,23 @@ static REAL load_val(const orc_solver *s, int i, REAL raw) {
 static int is_solid(const orc_solver *s, long c) { return s->has_ids && s->ids[c] != 0; }
 
 /* Equilibrium, second order, c_s^2 = 1/3 (collide.comp:589-606 with exact 3, 9/2, 3/2
- * as in set_equilibrium_populations.comp:179-196). */
-static REAL feq(REAL w, REAL rho, REAL cu, REAL uu) {
-    return w * rho * ((REAL)1 + (REAL)3 * cu + (REAL)4.5 * cu * cu - (REAL)1.5 * uu);
+ * as in set_equilibrium_populations.comp:179-196):
+ *   w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2) = (w rho) (c1 + cu (3 + 9/2 cu)),
+ * with c1 = 1 - 3/2 u^2 computed once per cell. */
+static REAL feq(REAL w, REAL rho, REAL cu, REAL c1) {
+    return (w * rho) * FMA(cu, FMA((REAL)4.5, cu, (REAL)3), c1);
 }
 
-/* Thermal equilibrium with a configurable c_s^2 (collide.comp:634-651). */
-static REAL geq(REAL w, REAL T, REAL cu, REAL uu, REAL cs2) {
-    return w * T * ((REAL)1 + cu / cs2 + cu * cu / ((REAL)2 * cs2 * cs2) - uu / ((REAL)2 * cs2));
+/* Thermal equilibrium with a configurable c_s^2 (collide.comp:634-651):
+ *   w T (1 + cu/cs2 + cu^2/(2 cs2^2) - u^2/(2 cs2)) = (w T) (t1 + cu (k1 + cu k2)),
+ * k1 = 1/cs2, k2 = k1^2/2, t1 = 1 - u^2 k1/2. */
+static REAL geq(REAL w, REAL T, REAL cu, REAL k1, REAL k2, REAL t1) {
+    return (w * T) * FMA(cu, FMA(cu, k2, k1), t1);
+}
+
+/* u.u = ((ux ux) + uy uy) + uz uz with FMAs */
+static REAL dot3(REAL ax, REAL ay, REAL az, REAL bx, REAL by, REAL bz) {
+    return FMA(az, bz, FMA(ay, by, ax * bx));
 }
 
 /* AA read of population i of the cell at (x,y,z) (collide.comp:244-305 + halfway BB). */
@@ -268,7 +285,7 @@ static REAL read_pop(const orc_solver *s, int x, int y, int z, int i, int odd) {
         const REAL *o = s->obj[s->ids[src]];
         REAL cu = (REAL)c[0] * o[0] + (REAL)c[1] * o[1] + (REAL)c[2] * o[2];
         /* f_i(x) = f*_opp(i)(x) - 2 w_opp rho_w (c_opp . u_w) * 3, rho_w = 1 (stream.comp:487-496,577-580) */
-        return load_val(s, opp(i), F_AT(s, opp(i), self)) + (REAL)6 * (REAL)s->T[i].w * cu;
+        return FMA((REAL)6 * (REAL)s->T[i].w, cu, load_val(s, opp(i), F_AT(s, opp(i), self)));
     }
     return load_val(s, i, F_AT(s, i, src));
 }
@@ -283,7 +300,7 @@ static void write_pop(orc_solver *s, int x, int y, int z, int i, int odd, REAL v
     if (s->bb_mode == 0 && is_solid(s, dst)) {
         const REAL *o = s->obj[s->ids[dst]];
         REAL cu = (REAL)c[0] * o[0] + (REAL)c[1] * o[1] + (REAL)c[2] * o[2];
-        F_AT(s, i, self) = store_val(s, i, v - (REAL)6 * (REAL)s->T[i].w * cu);
+        F_AT(s, i, self) = store_val(s, i, FMA(-(REAL)6 * (REAL)s->T[i].w, cu, v));
         return;
     }
     F_AT(s, opp(i), dst) = store_val(s, opp(i), v);
@@ -298,7 +315,7 @@ static REAL read_th(const orc_solver *s, int x, int y, int z, int k, int odd) {
     long src;
     if (!nbr(s, x, y, z, c, -1, &src)) return G_AT(s, opp(k), self);
     if (is_solid(s, src))
-        return -G_AT(s, opp(k), self) + (REAL)2 * (REAL)s->TT[k].w * s->obj[s->ids[src]][3];
+        return FMA((REAL)2 * (REAL)s->TT[k].w, s->obj[s->ids[src]][3], -G_AT(s, opp(k), self));
     return G_AT(s, k, src);
 }
 
@@ -309,7 +326,7 @@ static void write_th(orc_solver *s, int x, int y, int z, int k, int odd, REAL v)
     long dst;
     if (!nbr(s, x, y, z, c, +1, &dst)) { G_AT(s, k, self) = v; return; }
     if (is_solid(s, dst)) {
-        G_AT(s, k, self) = -v + (REAL)2 * (REAL)s->TT[k].w * s->obj[s->ids[dst]][3];
+        G_AT(s, k, self) = FMA((REAL)2 * (REAL)s->TT[k].w, s->obj[s->ids[dst]][3], -v);
         return;
     }
     G_AT(s, opp(k), dst) = v;
@@ -318,10 +335,10 @@ static void write_th(orc_solver *s, int x, int y, int z, int k, int odd, REAL v)
 /* psi = rho0 (1 - exp(-rho/rho0)), rho0 = 1 (collide.comp:512-516) */
 #ifdef ORACLE_F32
 static REAL psi_of(REAL rho) { return 1.0f - expf(-rho); }
-static REAL vlen(REAL a, REAL b, REAL c) { return sqrtf(a * a + b * b + c * c); }
+static REAL psqrt(REAL v) { return sqrtf(v); }
 #else
 static REAL psi_of(REAL rho) { return (REAL)1 - (REAL)exp(-(double)rho); }
-static REAL vlen(REAL a, REAL b, REAL c) { return (REAL)sqrt((double)(a * a + b * b + c * c)); }
+static REAL psqrt(REAL v) { return (REAL)sqrt((double)v); }
 #endif
 
 /* One tick (collide.comp:676-793). */
@@ -372,28 +389,33 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
             REAL k = -ps * s->G;
             F[0] += acc[0] * k; F[1] += acc[1] * k; F[2] += acc[2] * k;
         }
-        REAL u[3];                                        /* collide.comp:722-725 */
-        for (int a = 0; a < 3; ++a) u[a] = m[a] / rho + F[a] / ((REAL)2 * rho);
-        REAL un = vlen(u[0], u[1], u[2]);
+        /* u = m/rho + F/(2 rho) with one division (collide.comp:722-725) */
+        REAL inv_rho = (REAL)1 / rho, half_inv = (REAL)0.5 * inv_rho;
+        REAL u[3];
+        for (int a = 0; a < 3; ++a) u[a] = FMA(F[a], half_inv, m[a] * inv_rho);
+        REAL un = psqrt(dot3(u[0], u[1], u[2], u[0], u[1], u[2]));
         int ext = un > s->uext;
         if (un > s->ulim) { REAL sc = s->ulim / un; u[0] *= sc; u[1] *= sc; u[2] *= sc; }
         if (update) {                                     /* collide.comp:727-745 */
-            REAL uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            REAL fu = F[0] * u[0] + F[1] * u[1] + F[2] * u[2];
+            REAL uu = dot3(u[0], u[1], u[2], u[0], u[1], u[2]);
+            REAL fu = dot3(F[0], F[1], F[2], u[0], u[1], u[2]);
+            REAL c1 = FMA((REAL)-1.5, uu, (REAL)1);
+            REAL inv_tau = (REAL)1 / s->tau;
             REAL pre = (REAL)1 - (REAL)1 / ((REAL)2 * s->tau);
             int src_on = s->forcing || s->multiphase || s->thermal;
             for (int i = 0; i < s->q; ++i) {
                 const int *c = s->T[i].c;
                 REAL w = (REAL)s->T[i].w;
                 REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
-                REAL e = feq(w, rho, cu, uu);
+                REAL e = feq(w, rho, cu, c1);
                 REAL v;
                 if (ext) v = e;
                 else {
-                    v = f[i] - (f[i] - e) / s->tau;
+                    v = FMA(e - f[i], inv_tau, f[i]);   /* f - (f - feq)/tau */
                     if (src_on) {
                         REAL cf = (REAL)c[0] * F[0] + (REAL)c[1] * F[1] + (REAL)c[2] * F[2];
-                        v += pre * w * ((REAL)3 * cf + (REAL)9 * cf * cu - (REAL)3 * fu);
+                        /* Guo: (1 - 1/(2 tau)) w (3 cF + 9 cF cu - 3 F.u) */
+                        v += (pre * w) * (FMA((REAL)9 * cf, cu, (REAL)3 * cf) - (REAL)3 * fu);
                     }
                 }
                 write_pop(s, x, y, z, i, odd, v);
@@ -401,12 +423,15 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
         }
         if (s->multiphase) psi_cur[cell] = psi_of(rho);  /* collide.comp:763-769 (psi stored) */
         if (s->thermal) {                                 /* collide.comp:771-782 */
-            REAL uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            REAL uu = dot3(u[0], u[1], u[2], u[0], u[1], u[2]);
+            REAL k1 = (REAL)1 / s->cs2_t, k2 = (REAL)0.5 * k1 * k1;
+            REAL t1 = FMA(-(REAL)0.5 * k1, uu, (REAL)1);
+            REAL inv_tau_t = (REAL)1 / s->tau_t;
             for (int k = 0; k < s->qt; ++k) {
                 const int *c = s->TT[k].c;
                 REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
-                REAL e = geq((REAL)s->TT[k].w, T, cu, uu, s->cs2_t);
-                REAL v = g[k] - (g[k] - e) / s->tau_t;
+                REAL e = geq((REAL)s->TT[k].w, T, cu, k1, k2, t1);
+                REAL v = FMA(e - g[k], inv_tau_t, g[k]);
                 if (update) write_th(s, x, y, z, k, odd, v);
             }
         }
@@ -538,11 +563,12 @@ ORC_API int orc_init_fields(orc_solver *s, const double *rho, const double *u, c
         REAL r = (REAL)rho[c];
         REAL uu3[3] = {(REAL)u[3 * c], (REAL)u[3 * c + 1], (REAL)u[3 * c + 2]};
         if (s->has_ids && s->ids[c]) uu3[0] = uu3[1] = uu3[2] = 0;
-        REAL uu = uu3[0] * uu3[0] + uu3[1] * uu3[1] + uu3[2] * uu3[2];
+        REAL uu = dot3(uu3[0], uu3[1], uu3[2], uu3[0], uu3[1], uu3[2]);
+        REAL c1 = FMA((REAL)-1.5, uu, (REAL)1);
         for (int i = 0; i < s->q; ++i) {
             const int *cc = s->T[i].c;
             REAL cu = (REAL)cc[0] * uu3[0] + (REAL)cc[1] * uu3[1] + (REAL)cc[2] * uu3[2];
-            F_AT(s, i, c) = store_val(s, i, feq((REAL)s->T[i].w, r, cu, uu));
+            F_AT(s, i, c) = store_val(s, i, feq((REAL)s->T[i].w, r, cu, c1));
         }
     }
     free(s->g); s->g = NULL;
@@ -552,11 +578,13 @@ ORC_API int orc_init_fields(orc_solver *s, const double *rho, const double *u, c
             REAL T0 = temp ? (REAL)temp[c] : (REAL)1;
             REAL uu3[3] = {(REAL)u[3 * c], (REAL)u[3 * c + 1], (REAL)u[3 * c + 2]};
             if (s->has_ids && s->ids[c]) uu3[0] = uu3[1] = uu3[2] = 0;
-            REAL uu = uu3[0] * uu3[0] + uu3[1] * uu3[1] + uu3[2] * uu3[2];
+            REAL uu = dot3(uu3[0], uu3[1], uu3[2], uu3[0], uu3[1], uu3[2]);
+            REAL k1 = (REAL)1 / s->cs2_t, k2 = (REAL)0.5 * k1 * k1;
+            REAL t1 = FMA(-(REAL)0.5 * k1, uu, (REAL)1);
             for (int k = 0; k < s->qt; ++k) {
                 const int *cc = s->TT[k].c;
                 REAL cu = (REAL)cc[0] * uu3[0] + (REAL)cc[1] * uu3[1] + (REAL)cc[2] * uu3[2];
-                G_AT(s, k, c) = geq((REAL)s->TT[k].w, T0, cu, uu, s->cs2_t);
+                G_AT(s, k, c) = geq((REAL)s->TT[k].w, T0, cu, k1, k2, t1);
             }
         }
     }
