@@ -141,6 +141,21 @@ def cpu_baseline(n=64, ticks=150):
             "sample": f"D3Q19 TGV {n}^3 FP32, {ticks} ticks, oracle/liboracle_f32.so single thread ({dt:.1f} s)"}
 
 
+def counters_sample(bfa, n=256, ticks=10):
+    """Short workload for one rocprofv3 --pmc pass: a 1 GiB device copy of known
+    traffic (counter calibration), then `ticks` ticks of the FP32 and FP16 runs."""
+    import torch  # noqa: F401  (one HIP runtime with torch)
+    print("copy GB/s", round(bfa.capi.copy_bandwidth(1 << 30, 3), 1), flush=True)
+    rho, u = taylor_green(n)
+    for st in ("fp32", "fp16"):
+        s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19", storage=st)
+        s.init_fields(rho, u)
+        s.step(ticks)
+        s.synchronize()
+        s.close()
+    print("counters sample done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +166,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--counters-sample", action="store_true",
+                    help="run only the short workload used for rocprofv3 --pmc passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solver runs on a single MI355X; --gpus {args.gpus} is not supported",
@@ -158,6 +175,9 @@ def main():
         sys.exit(2)
     import boltzmannfluids_amd as bfa
     bfa.load_library()
+    if args.counters_sample:
+        counters_sample(bfa, args.size)
+        return
     n = args.size
     res = {st: run_gpu(bfa, n, st, args.steps, args.warmup) for st in (["fp32"] if args.fp32_only else ["fp32", "fp16", "bf16"])}
     main = res["fp32"]
