@@ -1,11 +1,18 @@
-"""Benchmark: D3Q19 BGK in-place (esoteric pull) tick on one MI355X.
+"""Benchmark: lattice Boltzmann ticks on one MI355X.
 
-Workload (default): 256^3 fully periodic D3Q19 box, Taylor-Green vortex
+Headline workload: 256^3 fully periodic D3Q19 box, Taylor-Green vortex
 (u0 = 0.05, rho = 1 + 3 p'), tau = 0.55, FP32 storage.  One step = one lattice
-tick over the whole grid; the populations stay resident in HBM.  Prints one
-JSON line: MLUPs/s of the FP32 run as `value`, the HBM roofline of the step
-kernel, the FP16/BF16 shifted-storage runs with their error against the FP64
-CPU oracle, and a CPU baseline (the FP32 CPU restatement, one core).
+tick over the whole grid; the populations stay resident in HBM.
+
+Prints one JSON line:
+  value / roofline        MLUPs/s of the FP32 run; HBM roofline of the tick kernel
+                          (algorithmic 152 B/cell / device-timed launch duration)
+  fp16_shifted, bf16_...  the same run with 16-bit shifted storage
+  error_vs_fp64           GPU fields vs the FP64 CPU oracle (same workload at 48^3)
+  other_configs           porous 384^3 D3Q19 (FP32/FP16/BF16, + error at 64^3),
+                          Shan-Chen D3Q27 + thermal D3Q7 192^3, D2Q9 cavity 256^2
+  adjoint_over_forward    differentiable channel: gradient time / forward time
+  cpu_baseline            the FP32 CPU restatement (oracle) on the host cores
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -20,29 +27,21 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from boltzmannfluids_amd.scenarios import (lid_cavity, porosity, porous_spheres,  # noqa: E402
+                                           spinodal, taylor_green)
+
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E published peak
-BYTES_PER_CELL = {"fp32": 2 * 19 * 4, "fp16": 2 * 19 * 2, "bf16": 2 * 19 * 2}  # 2*Q*s, no flags
+BYTES = {"fp32": 4, "fp16": 2, "bf16": 2}
 
 
-def taylor_green(n, u0=0.05, rho0=1.0):
-    k = 2 * np.pi / n
-    idx = np.arange(n, dtype=np.float64)
-    sx, cx = np.sin(k * idx), np.cos(k * idx)
-    c2 = np.cos(2 * k * idx)
-    # cell order z*n*n + y*n + x
-    ux = u0 * sx[None, None, :] * cx[None, :, None] * cx[:, None, None]
-    uy = -u0 * cx[None, None, :] * sx[None, :, None] * cx[:, None, None]
-    p = (rho0 * u0 ** 2 / 16.0) * (c2[None, None, :] + c2[None, :, None]) * (c2[:, None, None] + 2.0)
-    rho = (rho0 + 3.0 * p).astype(np.float32).ravel()
-    u = np.zeros((n ** 3, 3), np.float32)
-    u[:, 0] = ux.ravel()
-    u[:, 1] = uy.ravel()
-    return rho, u
+def bytes_per_cell(q, storage, extra=0):
+    """Algorithmic bytes per cell update: Q loads + Q stores (+ flags/aux)."""
+    return 2 * q * BYTES[storage] + extra
 
 
 def load_traffic(storage, size):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
-    a copy-kernel calibration) if a summary for this workload is committed."""
+    the copy-kernel calibration) if a summary for this workload is committed."""
     path = os.path.join(ROOT, "profiles", f"step_d3q19_{storage}_{size}_traffic.json")
     if not os.path.exists(path):
         return None
@@ -50,31 +49,47 @@ def load_traffic(storage, size):
         return json.load(fh).get("hbm_bytes_per_launch")
 
 
-def run_gpu(bfa, n, storage, steps, warmup):
+def roofline(bpc, cells, kernel_ms, traffic=None):
+    achieved = bpc * cells / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def timed_run(s, steps, warmup, conservation_every=0):
+    """Warm up, then time `steps` ticks with host wall clock (bracketed by device
+    synchronisation) and HIP events on the solver's stream."""
     import torch
-    rho, u = taylor_green(n)
-    s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19", storage=storage)
-    s.init_fields(rho, u)
     s.step(warmup)
     s.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev_ms = s.step_timed(steps)
+    if conservation_every:
+        ev = 0.0
+        for k in range(0, steps, conservation_every):
+            ev += s.step_timed(min(conservation_every, steps - k), conservation=True)
+    else:
+        ev = s.step_timed(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    cells = n ** 3
-    out = {
-        "mlups": cells * steps / wall / 1e6,
-        "ms_per_step": wall * 1e3 / steps,
-        "kernel_ms": ev_ms / steps,
-    }
-    b = BYTES_PER_CELL[storage] * cells
-    achieved = b / (out["kernel_ms"] * 1e-3) / 1e9
-    traffic = load_traffic(storage, n)
-    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+    return wall, ev
+
+
+def run_tgv(bfa, n, storage, steps, warmup):
+    rho, u = taylor_green(n)
+    s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19", storage=storage)
+    s.init_fields(rho, u)
+    wall, ev = timed_run(s, steps, warmup)
     s.close()
-    return out
+    cells = n ** 3
+    kernel_ms = ev / steps
+    return {"mlups": cells * steps / wall / 1e6, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kernel_ms,
+            "roofline": roofline(bytes_per_cell(19, storage), cells, kernel_ms, load_traffic(storage, n))}
+
+
+def field_error(a, b):
+    er = float(np.linalg.norm(a[:, 3] - b[:, 3]) / np.linalg.norm(b[:, 3]))
+    eu = float(np.linalg.norm(a[:, :3] - b[:, :3]) / np.linalg.norm(b[:, :3]))
+    return er, eu
 
 
 def accuracy_vs_fp64(bfa, storage, n=48, ticks=200):
@@ -91,23 +106,91 @@ def accuracy_vs_fp64(bfa, storage, n=48, ticks=200):
     o = oracle.Oracle((n, n, n), 0.55, velocity_set="D3Q19", precision="f64")
     o.init_fields(rho, u)
     o.step(ticks, save=True)
-    b = o.read("velocity_density")
-    er = float(np.linalg.norm(a[:, 3] - b[:, 3]) / np.linalg.norm(b[:, 3]))
-    eu = float(np.linalg.norm(a[:, :3] - b[:, :3]) / np.linalg.norm(b[:, :3]))
+    er, eu = field_error(a, o.read("velocity_density"))
     return {"rho": er, "u": eu, "workload": f"TGV {n}^3 tau=0.55 u0=0.05, {ticks} ticks"}
 
 
-def porosity(X, Y, Z=1, seed=20261203, scale=0.02):
-    """0.02 x (3x3 box-smoothed seeded U[0,1]), zero on the wall rows."""
-    rng = np.random.default_rng(seed)
-    a = rng.random((Z, Y, X))
-    k = np.ones(3) / 3.0
-    for ax in ((1, 2) if Z == 1 else (0, 1, 2)):
-        a = np.apply_along_axis(lambda v: np.convolve(np.pad(v, 1, mode="edge"), k, mode="valid"), ax, a)
-    g = np.clip(scale * a, 0.0, 1.0)
-    g[:, 0, :] = 0.0
-    g[:, -1, :] = 0.0
-    return g.ravel().astype(np.float32)
+def porous_config(bfa, n=384, steps=100, warmup=10, err_n=64, err_ticks=300):
+    """D3Q19 porous duct: periodic x, walls on y/z, 5 % solid spheres r in [4, 12],
+    g_x = 1e-6, tau = 0.56; FP32/FP16/BF16 throughput at n^3, field error vs
+    the FP64 oracle at err_n^3 (same generator, same parameters)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    out = {"workload": f"D3Q19 {n}^3 porous duct (periodic x, walls y/z, 5% spheres r 4-12), g_x 1e-6, tau 0.56",
+           "storage": {}}
+    ids = porous_spheres(n, n, n)
+    cells = n ** 3
+    fluid = int(np.count_nonzero(ids == 0))
+    out["solid_fraction"] = round(1 - fluid / cells, 4)
+    rho = np.ones(cells, np.float32)
+    u = np.zeros((cells, 3), np.float32)
+    force = np.array([1e-6, 0, 0], np.float32)
+    small_ids = porous_spheres(err_n, err_n, err_n)
+    sc = err_n ** 3
+    ref = oracle.Oracle((err_n,) * 3, 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", precision="f64")
+    ref.init_fields(np.ones(sc, np.float32), np.zeros((sc, 3), np.float32), force=np.tile(force, (sc, 1)),
+                    boundary_id=small_ids)
+    ref.step(err_ticks, save=True)
+    ref_vd = ref.read("velocity_density")
+    for st in ("fp32", "fp16", "bf16"):
+        s = bfa.Solver((n, n, n), 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", storage=st)
+        s.init_fields(rho, u, boundary_id=ids)
+        s.set_param("constant_force", force)
+        wall, ev = timed_run(s, steps, warmup)
+        s.close()
+        kms = ev / steps
+        g = bfa.Solver((err_n,) * 3, 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", storage=st)
+        g.init_fields(np.ones(sc, np.float32), np.zeros((sc, 3), np.float32), force=np.tile(force, (sc, 1)),
+                      boundary_id=small_ids)
+        g.step(err_ticks, save=True)
+        vd = g.read("velocity_density").astype(np.float64)
+        g.close()
+        mask = small_ids == 0
+        er, eu = field_error(vd[mask], ref_vd[mask])
+        out["storage"][st] = {"mlups": round(cells * steps / wall / 1e6, 1),
+                              "fluid_mlups": round(fluid * steps / wall / 1e6, 1),
+                              "roofline": roofline(bytes_per_cell(19, st, 1), cells, kms),
+                              "error_vs_fp64": {"rho": er, "u": eu, "workload": f"{err_n}^3, {err_ticks} ticks"}}
+    return out
+
+
+def spinodal_thermal_config(bfa, n=192, steps=100, warmup=10):
+    """D3Q27 Shan-Chen (G = -5, psi = 1 - exp(-rho), rho = 2 + U(+-0.01), tau = 1)
+    with a D3Q7 thermal lattice: hot wall T = 1 at y = 0, cold wall T = 0 at
+    y = n-1, Boussinesq buoyancy with gravity (0, -1e-5, 0), T_ref = 0.5;
+    FP64 mass sums every 100 ticks inside the timed region."""
+    rho, u, ids = spinodal(n, n, n)
+    cells = n ** 3
+    s = bfa.Solver((n, n, n), 1.0, periodic=(1, 0, 1), velocity_set="D3Q27", multiphase=True)
+    s.set_object(1, temperature=1.0)
+    s.set_object(2, temperature=0.0)
+    s.set_param("interaction_strength", -5.0)
+    s.set_param("reference_temperature", 0.5)
+    T = np.full(cells, 0.5, np.float32)
+    force = np.tile(np.array([0, -1e-5, 0], np.float32), (cells, 1))
+    s.init_fields(rho, u, force=force, temperature=T, boundary_id=ids)
+    m0 = s.conservation()[0]
+    wall, ev = timed_run(s, steps, warmup, conservation_every=100)
+    m1 = s.last_step_sums()[0]
+    s.close()
+    kms = ev / steps
+    return {"workload": f"D3Q27 Shan-Chen {n}^3 (G -5, tau 1) + D3Q7 thermal walls, FP32",
+            "mlups": round(cells * steps / wall / 1e6, 1),
+            "roofline": roofline(2 * 27 * 4 + 2 * 7 * 4 + 8 + 1, cells, kms),
+            "fp64_mass_relative_drift": abs(m1 - m0) / m0}
+
+
+def cavity_config(bfa, n=256, steps=1000, warmup=50):
+    """D2Q9 lid-driven cavity, tau 0.6, lid u 0.1, halfway bounce-back walls (2.4 MB:
+    cache resident, reported as time per tick)."""
+    s = bfa.Solver((n, n, 1), 0.6, periodic=(0, 0, 1), velocity_set="D2Q9")
+    s.set_object(2, velocity=(0.1, 0.0, 0.0))
+    cells = n * n
+    s.init_fields(np.ones(cells, np.float32), np.zeros((cells, 3), np.float32), boundary_id=lid_cavity(n, n))
+    wall, ev = timed_run(s, steps, warmup)
+    s.close()
+    return {"workload": f"D2Q9 lid-driven cavity {n}^2, tau 0.6, lid 0.1", "us_per_tick": round(wall * 1e6 / steps, 2),
+            "kernel_us_per_tick": round(ev * 1e3 / steps, 2), "mlups": round(cells * steps / wall / 1e6, 1)}
 
 
 def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
@@ -128,17 +211,19 @@ def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
 
 
 def cpu_baseline(n=64, ticks=150):
-    """FP32 CPU restatement (oracle/liboracle_f32.so), one thread, bounded sample."""
+    """FP32 CPU restatement (oracle/liboracle_f32.so) on the host cores, bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     rho, u = taylor_green(n)
     o = oracle.Oracle((n, n, n), 0.55, velocity_set="D3Q19", precision="f32")
     o.init_fields(rho, u)
+    threads = int(oracle.load("f32").orc_threads())
     t0 = time.perf_counter()
     o.step(ticks)
     dt = time.perf_counter() - t0
-    return {"value": round(n ** 3 * ticks / dt / 1e6, 3), "unit": "MLUPs/s", "cores": 1, "kind": "port",
-            "sample": f"D3Q19 TGV {n}^3 FP32, {ticks} ticks, oracle/liboracle_f32.so single thread ({dt:.1f} s)"}
+    return {"value": round(n ** 3 * ticks / dt / 1e6, 3), "unit": "MLUPs/s", "cores": threads, "kind": "port",
+            "sample": f"D3Q19 TGV {n}^3 FP32, {ticks} ticks, oracle/liboracle_f32.so, {threads} OpenMP threads "
+                      f"({dt:.1f} s)"}
 
 
 def counters_sample(bfa, n=256, ticks=10):
@@ -166,6 +251,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--counters-sample", action="store_true",
                     help="run only the short workload used for rocprofv3 --pmc passes")
     args = ap.parse_args()
@@ -179,33 +265,36 @@ def main():
         counters_sample(bfa, args.size)
         return
     n = args.size
-    res = {st: run_gpu(bfa, n, st, args.steps, args.warmup) for st in (["fp32"] if args.fp32_only else ["fp32", "fp16", "bf16"])}
-    main = res["fp32"]
+    stores = ["fp32"] if args.fp32_only else ["fp32", "fp16", "bf16"]
+    res = {st: run_tgv(bfa, n, st, args.steps, args.warmup) for st in stores}
+    main_run = res["fp32"]
     line = {
         "metric": "D3Q19 MLUPs/s & %HBM peak, FP16-store vs FP32 @<=1e-3 rel err; adjoint/fwd time",
-        "value": round(main["mlups"], 1),
+        "value": round(main_run["mlups"], 1),
         "unit": "MLUPs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(main["ms_per_step"], 5),
+        "ms_per_step": round(main_run["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (Taylor-Green vortex initial state)",
         "config": {"workload": f"D3Q19 BGK esoteric-pull, {n}^3 fully periodic, TGV u0=0.05, tau=0.55, FP32 storage",
-                   "cells": n ** 3, "bytes_per_cell": BYTES_PER_CELL["fp32"]},
-        "roofline": main["roofline"],
+                   "cells": n ** 3, "bytes_per_cell": bytes_per_cell(19, "fp32")},
+        "roofline": main_run["roofline"],
     }
     for st in ("fp16", "bf16"):
         if st in res:
             r = res[st]
             line[f"{st}_shifted"] = {"value": round(r["mlups"], 1), "unit": "MLUPs/s",
-                                     "speedup_vs_fp32": round(r["mlups"] / main["mlups"], 3),
+                                     "speedup_vs_fp32": round(r["mlups"] / main_run["mlups"], 3),
                                      "roofline": r["roofline"]}
     if not args.no_accuracy:
         line["error_vs_fp64"] = {st: accuracy_vs_fp64(bfa, st) for st in res}
+    if not args.no_other_configs:
+        line["other_configs"] = [porous_config(bfa), spinodal_thermal_config(bfa), cavity_config(bfa)]
     if not args.no_adjoint:
         line["adjoint_over_forward"] = [adjoint_ratio(bfa, (512, 128, 1), "D2Q9"),
                                         adjoint_ratio(bfa, (128, 128, 128), "D3Q19")]

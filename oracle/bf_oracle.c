@@ -39,6 +39,9 @@
  * profile, bit-pack round trip.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -346,11 +349,14 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
     int odd = (int)(s->t & 1);
     const REAL *psi_prev = s->multiphase ? s->psi[s->cur ^ 1] : NULL;
     REAL *psi_cur = s->multiphase ? s->psi[s->cur] : NULL;
-    double sm[5] = {0, 0, 0, 0, 0};
-    REAL f[27], g[7];
-    for (int z = 0; z < s->Z; ++z)
-    for (int y = 0; y < s->Y; ++y)
-    for (int x = 0; x < s->X; ++x) {
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+    /* each cell reads exactly the slots it writes (AA pattern), so cells are
+     * independent and the loop runs in parallel when built with OpenMP */
+    const long ncell = (long)s->X * s->Y * s->Z;
+#pragma omp parallel for schedule(static) reduction(+ : s0, s1, s2, s3, s4)
+    for (long cc = 0; cc < ncell; ++cc) {
+        REAL f[27], g[7];
+        int x = (int)(cc % s->X), y = (int)((cc / s->X) % s->Y), z = (int)(cc / ((long)s->X * s->Y));
         long cell = cid(s, x, y, z);
         if (is_solid(s, cell)) continue;                 /* collide.comp:696-700 */
         for (int i = 0; i < s->q; ++i) f[i] = read_pop(s, x, y, z, i, odd);
@@ -361,7 +367,7 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
             m[1] += f[i] * (REAL)s->T[i].c[1];
             m[2] += f[i] * (REAL)s->T[i].c[2];
         }
-        if (want_sums) { sm[0] += (double)rho; sm[1] += (double)m[0]; sm[2] += (double)m[1]; sm[3] += (double)m[2]; }
+        if (want_sums) { s0 += (double)rho; s1 += (double)m[0]; s2 += (double)m[1]; s3 += (double)m[2]; }
         if (rho < (REAL)0.01) rho = (REAL)0.01;          /* collide.comp:705 */
         REAL F[3] = {0, 0, 0};
         if (s->forcing == 1) { F[0] = s->fconst[0]; F[1] = s->fconst[1]; F[2] = s->fconst[2]; }
@@ -370,7 +376,7 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
         if (s->thermal) {                                 /* collide.comp:709-716 */
             for (int k = 0; k < s->qt; ++k) g[k] = read_th(s, x, y, z, k, odd);
             for (int k = 0; k < s->qt; ++k) T += g[k];
-            if (want_sums) sm[4] += (double)T;
+            if (want_sums) s4 += (double)T;
             REAL b = -s->beta * (T - s->t_ref);
             REAL Fb[3] = {b * F[0], b * F[1], b * F[2]};
             F[0] += Fb[0]; F[1] += Fb[1]; F[2] += Fb[2];
@@ -442,7 +448,7 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
             s->ft[4 * cell + 2] = (float)F[2]; s->ft[4 * cell + 3] = (float)T;
         }
     }
-    if (want_sums) memcpy(s->sums, sm, sizeof sm);
+    if (want_sums) { s->sums[0] = s0; s->sums[1] = s1; s->sums[2] = s2; s->sums[3] = s3; s->sums[4] = s4; }
     if (s->multiphase) s->cur ^= 1;
 }
 
@@ -460,6 +466,15 @@ ORC_API int orc_velocity_table(int set, int thermal, double *out) {
 }
 
 ORC_API int orc_opposite(int i) { return opp(i); }
+
+/* Threads the parallel loops use (1 without OpenMP). */
+ORC_API int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 ORC_API orc_solver *orc_create(int X, int Y, int Z, int vset, int storage, int multiphase,
                                int bb_mode, int px, int py, int pz, double tau) {

@@ -37,6 +37,7 @@ def load(precision="f64"):
     sig = {
         "orc_velocity_table": (i32, [i32, i32, dp]),
         "orc_opposite": (i32, [i32]),
+        "orc_threads": (i32, []),
         "orc_create": (vp, [i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, C.c_double]),
         "orc_destroy": (None, [vp]),
         "orc_set_param": (i32, [vp, i32, dp, i32]),
