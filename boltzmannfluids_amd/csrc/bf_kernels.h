@@ -26,8 +26,10 @@ struct StepArgs {
     int X, Y, Z;
     int periodic_x, periodic_y, periodic_z;
     const uint8_t *ids;      // boundary id per cell (or null)
+    const uint32_t *nmask;   // solid-neighbour mask per cell (see launch_mask), with ids
     const ObjectEntry *objects;  // 256 entries
     int halfway;             // 1: halfway bounce-back redirect, 0: full-way (solid cells skipped only)
+    int wall_ids;            // wall terms need the neighbours' ids (moving walls or thermal walls)
     int forcing;             // 0 none, 1 constant, 2 per-cell field
     float fx, fy, fz;        // constant force
     const float4 *force;     // per-cell force (xyz) when forcing == 2
@@ -95,6 +97,14 @@ int x_direction_count(int velocity_set);
 template <int VS>
 hipError_t launch_step(const StepArgs &a, int storage, bool odd, bool walls, bool multiphase,
                        bool thermal, const LaunchShape &shape, double *partial, hipStream_t stream);
+
+// Neighbour mask of every cell: bit 0 = the cell is solid, bit i (1..Q-1) =
+// the neighbour at x + c_i lies inside the domain (periodic axes wrap) and is
+// solid.  Built once per boundary field; the even tick selects its bounce-back
+// addresses from it with one 4-byte load instead of Q-1 neighbour id loads.
+template <int VS>
+hipError_t launch_mask(const uint8_t *ids, uint32_t *nmask, int X, int Y, int Z, int px, int py, int pz,
+                       hipStream_t stream);
 
 template <int VS>
 hipError_t launch_init(const InitArgs &a, int storage, bool thermal, hipStream_t stream);

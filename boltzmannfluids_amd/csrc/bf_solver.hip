@@ -37,6 +37,7 @@ struct bf_solver {
     void *f = nullptr;
     float *g = nullptr;
     uint8_t *ids = nullptr;
+    uint32_t *nmask = nullptr;  // solid-neighbour masks (with ids)
     ObjectEntry *objects_d = nullptr;
     float *rho_eff_d = nullptr;
     float4 *force_d = nullptr;
@@ -126,6 +127,17 @@ hipError_t dispatch_step(const bf_solver *s, const StepArgs &a, bool odd, double
     return hipErrorInvalidValue;
 }
 
+hipError_t dispatch_mask(const bf_solver *s) {
+    const int px = s->cfg.periodic[0] != 0, py = s->cfg.periodic[1] != 0, pz = s->cfg.periodic[2] != 0;
+    switch (s->vs) {
+    case D2Q9: return launch_mask<D2Q9>(s->ids, s->nmask, s->X, s->Y, s->Z, px, py, pz, s->stream);
+    case D3Q15: return launch_mask<D3Q15>(s->ids, s->nmask, s->X, s->Y, s->Z, px, py, pz, s->stream);
+    case D3Q19: return launch_mask<D3Q19>(s->ids, s->nmask, s->X, s->Y, s->Z, px, py, pz, s->stream);
+    case D3Q27: return launch_mask<D3Q27>(s->ids, s->nmask, s->X, s->Y, s->Z, px, py, pz, s->stream);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t dispatch_init(const bf_solver *s, const InitArgs &a) {
     const bool th = s->thermal != 0;
     switch (s->vs) {
@@ -166,8 +178,13 @@ StepArgs make_args(bf_solver *s, bool update) {
     a.periodic_y = s->cfg.periodic[1] != 0;
     a.periodic_z = s->cfg.periodic[2] != 0;
     a.ids = s->ids;
+    a.nmask = s->nmask;
     a.objects = s->objects_d;
     a.halfway = s->cfg.boundary_mode == BF_BB_HALFWAY;
+    // wall terms read the solid neighbour's object only when one can be non-zero
+    a.wall_ids = s->thermal ? 1 : 0;
+    for (int i = 1; i < 256 && a.halfway && !a.wall_ids; ++i)
+        if (s->objects_h[i].ux != 0.0f || s->objects_h[i].uy != 0.0f || s->objects_h[i].uz != 0.0f) a.wall_ids = 1;
     a.forcing = s->forcing;
     a.fx = s->fconst[0]; a.fy = s->fconst[1]; a.fz = s->fconst[2];
     a.force = s->force_d;
@@ -347,7 +364,7 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
 void bf_destroy(bf_solver *s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    dfree(s->f); dfree(s->g); dfree(s->ids); dfree(s->objects_d); dfree(s->rho_eff_d);
+    dfree(s->f); dfree(s->g); dfree(s->ids); dfree(s->nmask); dfree(s->objects_d); dfree(s->rho_eff_d);
     dfree(s->force_d); dfree(s->psi[0]); dfree(s->psi[1]); dfree(s->vd); dfree(s->ft);
     dfree(s->partial); dfree(s->sums_d);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -395,6 +412,7 @@ bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const f
 
     // boundaries (LBM.cpp:593-691): ids grow the object table like objects_cpu.resize
     dfree(s->ids); s->ids = nullptr;
+    dfree(s->nmask); s->nmask = nullptr;
     s->has_ids = 0;
     if (boundary_id) {
         int maxid = 0;
@@ -405,6 +423,8 @@ bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const f
             bf_status st;
             if ((st = dalloc(s, &s->ids, (size_t)N)) != BF_OK) return st;
             BF_HIP(s, hipMemcpy(s->ids, boundary_id, (size_t)N, hipMemcpyHostToDevice));
+            if ((st = dalloc(s, &s->nmask, sizeof(uint32_t) * (size_t)N)) != BF_OK) return st;
+            BF_HIP(s, dispatch_mask(s));
         }
     }
     // force detection (LBM.cpp:531-541,562)

@@ -151,7 +151,6 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     static_assert(CPT == 1 || (!MP && !TH), "multi-cell threads only for the plain and wall kernels");
     static_assert(!(AL && (ODD || MP)), "aligned rows only for the even tick of non-multiphase kernels");
     static_assert(!(PER && (WALLS || MP || TH)), "the periodic specialisation is the plain kernel");
-    constexpr int NXD = x_dir_count<VS>();
     extern __shared__ float lds_rows[];
     using L = Lattice<VS>;
     using S = Store<ST>;
@@ -215,71 +214,74 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         __syncthreads();
     }
 
-    if (WALLS) {
+    // Solid cells skip (collide.comp:696-700).  Even ticks take the solid-neighbour
+    // mask (bit 0 = self solid, bit j = neighbour at x + c_j solid), odd ticks
+    // only need the cell's own id.
+    uint32_t nm[CPT];
 #pragma unroll
-        for (int k = 0; k < CPT; ++k)
-            if (active[k]) active[k] = a.ids[self[k]] == 0;  // solid cells skip (collide.comp:696-700)
+    for (int k = 0; k < CPT; ++k) {
+        nm[k] = 0u;
+        if (WALLS && active[k]) {
+            if constexpr (ODD) {
+                active[k] = a.ids[self[k]] == 0;
+            } else {
+                nm[k] = a.nmask[self[k]];
+                active[k] = (nm[k] & 1u) == 0u;
+            }
+        }
     }
+#define SOLID(k, j) (WALLS && !ODD && ((nm[k] >> (j)) & 1u))
 
     double s_rho = 0.0, s_mx = 0.0, s_my = 0.0, s_mz = 0.0, s_t = 0.0;
 
-    // ---- phase 1: boundary ids of the neighbours and AA reads (collide.comp:244-305) ----
-    // 1a loads the neighbour ids that select addresses, 1b issues every
-    // population load of the tick, 1c converts and applies the open-boundary
-    // and bounce-back rules.  No loaded value is used before 1c.
-    uint8_t nb[CPT][Q];
+    // ---- phase 1: AA reads (collide.comp:244-305) ----
+    // 1b issues every population load of the tick (addresses from the
+    // neighbour mask), then the ids of solid neighbours where wall terms need
+    // them; 1c converts and applies the open-boundary and bounce-back rules.
+    // No loaded value is used before 1c.
     T raw[CPT][Q];    // x-directions under AL: the value staged for the row buffer
-    T rawr[CPT][Q];   // x-directions under AL: own opposite slot for a wall redirect
     float graw[QT];
     float fi[CPT][Q];
     float gi[QT];
-    const bool need_nb = WALLS && !ODD && (a.halfway || TH);
     const int lds_plane = (int)blockDim.y * X;  // row buffer: [x-direction][row of the block][x]
     const int lds_base = ty * X;
 #define LDS(i, col) lds_rows[x_dir_index<VS>(i) * lds_plane + lds_base + (col)]
 
-    // 1a
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        if (!active[k] || !need_nb) continue;
-#pragma unroll
-        for (int i = 1; i < Q; ++i) {
-            const int dx = L::c[i][0] + 1, dy = L::c[i][1] + 1, dz = L::c[i][2] + 1;
-            nb[k][i] = OK(k, dx, dy, dz) ? a.ids[xs[k][dx] + ry[dy] + pz[dz]] : 0;
-        }
-    }
-    // 1b
+    // 1b.  Cells that do not update (solid, or past the grid) load cell 0 of
+    // the same slots: one shared line per wave, and no branch splits the
+    // batch of loads (a branch would let the compiler consume the first loads
+    // before issuing the rest).
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         if constexpr (AL) {
             // every cell of the block's rows loads, at its own x, slot i of the
             // row the cell at x + c_x pulls from
-            if (xs[k][1] < X && y < Y) {
+            const bool in_grid = xs[k][1] < X && y < Y;
 #pragma unroll
-                for (int i = 1; i < Q; ++i) {
-                    if (L::c[i][0] == 0) continue;
-                    const int dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
-                    const int idx = (PER || (vy[dy] && vz[dz])) ? xs[k][1] + ry[dy] + pz[dz] : self[k];
-                    raw[k][i] = S::raw(f + (size_t)i * a.stride, idx);
-                }
+            for (int i = 1; i < Q; ++i) {
+                if (L::c[i][0] == 0) continue;
+                const int dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
+                const int idx = !in_grid ? 0 : (PER || (vy[dy] && vz[dz])) ? xs[k][1] + ry[dy] + pz[dz] : self[k];
+                raw[k][i] = S::raw(f + (size_t)i * a.stride, idx);
             }
         }
-        if (!active[k]) continue;
+        const bool act = active[k];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int o = opposite(i);
             if (ODD || i == 0) {
-                raw[k][i] = S::raw(f + (size_t)(ODD ? o : 0) * a.stride, self[k]);
+                raw[k][i] = S::raw(f + (size_t)(ODD ? o : 0) * a.stride, act ? self[k] : 0);
                 continue;
             }
             const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
             const bool valid = vx[k][dx] && vy[dy] && vz[dz];
-            const bool redir = WALLS && a.halfway && valid && nb[k][o] != 0;
+            const bool redir = a.halfway && SOLID(k, o);
             if (AL && L::c[i][0] != 0) {
-                if (redir) rawr[k][i] = S::raw(f + (size_t)o * a.stride, self[k]);
+                // own opposite slot for a wall redirect: loaded below
             } else {
-                const int idx = (valid && !redir) ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
-                raw[k][i] = S::raw(f + (size_t)(redir ? o : i) * a.stride, idx);
+                const int idx = !act ? 0 : (valid && !redir) ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
+                T *const pi = f + (size_t)i * a.stride, *const po = f + (size_t)o * a.stride;
+                raw[k][i] = S::raw(redir ? po : pi, idx);
             }
         }
         if constexpr (TH) {
@@ -287,15 +289,19 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int q = 0; q < QT; ++q) {
                 const int o = opposite(q);
                 if (ODD || q == 0) {
-                    graw[q] = a.g[(size_t)(ODD ? o : 0) * a.stride + self[k]];
+                    graw[q] = a.g[(size_t)(ODD ? o : 0) * a.stride + (act ? self[k] : 0)];
                     continue;
                 }
                 const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
-                const bool pull = OK(k, dx, dy, dz) && !(WALLS && nb[k][o] != 0);
-                graw[q] = a.g[(size_t)(pull ? q : o) * a.stride + (pull ? xs[k][dx] + ry[dy] + pz[dz] : self[k])];
+                const bool pull = OK(k, dx, dy, dz) && !SOLID(k, o);
+                const int idx = !act ? 0 : pull ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
+                graw[q] = a.g[(size_t)(pull ? q : o) * a.stride + idx];
             }
         }
     }
+    // object of the solid neighbour at x + c_j (looked up only where a wall
+    // term can be non-zero: a moving wall or a thermal wall)
+#define WALL(k, j) a.objects[a.ids[xs[k][1 + L::c[j][0]] + ry[1 + L::c[j][1]] + pz[1 + L::c[j][2]]]]
     // 1c
     if constexpr (AL) {
 #pragma unroll
@@ -321,13 +327,20 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             }
             const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
             const bool valid = vx[k][dx] && vy[dy] && vz[dz];
-            const bool redir = WALLS && a.halfway && valid && nb[k][o] != 0;
+            const bool redir = a.halfway && SOLID(k, o);
             if (!valid) {
                 fi[k][i] = 0.01f;  // open boundary read (collide.comp:297-298)
             } else if (redir) {
-                const ObjectEntry ob = a.objects[nb[k][o]];
-                const float cu = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
-                fi[k][i] = __builtin_fmaf(6.0f * w, cu, S::cvt((AL && L::c[i][0] != 0) ? rawr[k][i] : raw[k][i], L::w[o]));
+                // moving wall (Ladd): + 6 w rho_w (c_i . u_w), rho_w = 1; a wall at
+                // rest adds exactly zero.  (Aligned x-directions: after the row exchange.)
+                if (!(AL && L::c[i][0] != 0)) {
+                    float v = S::cvt(raw[k][i], L::w[o]);
+                    if (a.wall_ids) {
+                        const ObjectEntry ob = WALL(k, o);
+                        v = __builtin_fmaf(6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
+                    }
+                    fi[k][i] = v;
+                }
             } else if (!(AL && L::c[i][0] != 0)) {
                 fi[k][i] = S::cvt(raw[k][i], w);
             }
@@ -338,9 +351,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 gi[q] = graw[q];
                 if (ODD || q == 0) continue;
                 const int o = opposite(q);
-                const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
-                if (OK(k, dx, dy, dz) && WALLS && nb[k][o] != 0)
-                    gi[q] = __builtin_fmaf(2.0f * L::wt[q], a.objects[nb[k][o]].temperature, -graw[q]);
+                if (SOLID(k, o)) gi[q] = __builtin_fmaf(2.0f * L::wt[q], WALL(k, o).temperature, -graw[q]);
             }
         }
     }
@@ -355,7 +366,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 if (L::c[i][0] == 0) continue;
                 const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
                 if (!OK(k, dx, dy, dz)) continue;
-                if (WALLS && a.halfway && nb[k][opposite(i)] != 0) continue;
+                if (a.halfway && SOLID(k, opposite(i))) continue;
                 fi[k][i] = LDS(i, xs[k][dx]);
             }
         }
@@ -371,6 +382,34 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 const int dx = 1 + L::c[i][0];
                 if (!(PER || vx[k][dx])) continue;
                 LDS(i, xs[k][dx]) = __uint_as_float(kNoValue);
+            }
+        }
+        if constexpr (WALLS) {
+            // x-directions whose source is solid read the cell's own opposite
+            // slot.  A wave with such a cell issues all these loads together
+            // (cells without that redirect load cell 0), then converts them.
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                if (__builtin_amdgcn_ballot_w64(active[k] && a.halfway && (nm[k] >> 1) != 0u) == 0) continue;
+                T rr[Q];
+#pragma unroll
+                for (int i = 1; i < Q; ++i) {
+                    if (L::c[i][0] == 0) continue;
+                    const int o = opposite(i);
+                    rr[i] = S::raw(f + (size_t)o * a.stride, (active[k] && a.halfway && SOLID(k, o)) ? self[k] : 0);
+                }
+#pragma unroll
+                for (int i = 1; i < Q; ++i) {
+                    if (L::c[i][0] == 0) continue;
+                    const int o = opposite(i);
+                    if (!(active[k] && a.halfway && SOLID(k, o))) continue;
+                    float v = S::cvt(rr[i], L::w[o]);
+                    if (a.wall_ids) {
+                        const ObjectEntry ob = WALL(k, o);
+                        v = __builtin_fmaf(6.0f * L::w[i], dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
+                    }
+                    fi[k][i] = v;
+                }
             }
         }
     }
@@ -474,10 +513,13 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                     if (!OK(k, dx, dy, dz)) {
                         // open boundary: write dropped (collide.comp:358-359)
-                    } else if (WALLS && a.halfway && nb[k][i] != 0) {
-                        const ObjectEntry ob = a.objects[nb[k][i]];
-                        const float cw = dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz);
-                        S::store(f + (size_t)i * a.stride, self[k], __builtin_fmaf(-6.0f * w, cw, v), w);
+                    } else if (a.halfway && SOLID(k, i)) {
+                        float vb = v;
+                        if (a.wall_ids) {
+                            const ObjectEntry ob = WALL(k, i);
+                            vb = __builtin_fmaf(-6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
+                        }
+                        S::store(f + (size_t)i * a.stride, self[k], vb, w);
                     } else if (AL && L::c[i][0] != 0) {
                         LDS(i, xs[k][dx]) = v;
                     } else {
@@ -511,9 +553,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                         const int dx = 1 + L::c[q][0], dy = 1 + L::c[q][1], dz = 1 + L::c[q][2];
                         if (!OK(k, dx, dy, dz)) {
                             a.g[(size_t)q * a.stride + self[k]] = v;
-                        } else if (WALLS && nb[k][q] != 0) {
+                        } else if (SOLID(k, q)) {
                             a.g[(size_t)q * a.stride + self[k]] =
-                                __builtin_fmaf(2.0f * L::wt[q], a.objects[nb[k][q]].temperature, -v);
+                                __builtin_fmaf(2.0f * L::wt[q], WALL(k, q).temperature, -v);
                         } else {
                             a.g[(size_t)o * a.stride + xs[k][dx] + ry[dy] + pz[dz]] = v;
                         }
@@ -549,6 +591,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 
 #undef LDS
 #undef OK
+#undef SOLID
+#undef WALL
     // ---- FP64 block partial sums (deterministic two-stage reduction) ----
     if (partial) {
 #pragma unroll
@@ -619,6 +663,25 @@ __global__ void __launch_bounds__(256) k_init(const InitArgs a) {
     }
 }
 
+template <int VS>
+__global__ void __launch_bounds__(256) k_mask(const uint8_t *__restrict__ ids, uint32_t *__restrict__ nmask, int X,
+                                              int Y, int Z, int px, int py, int pz) {
+    using L = Lattice<VS>;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (int64_t)X * Y * Z) return;
+    const int x = (int)(c % X), y = (int)((c / X) % Y), z = (int)(c / ((int64_t)X * Y));
+    uint32_t m = ids[c] != 0 ? 1u : 0u;
+#pragma unroll
+    for (int j = 1; j < L::Q; ++j) {
+        int nx = x + L::c[j][0], ny = y + L::c[j][1], nz = z + L::c[j][2];
+        if (nx < 0 || nx >= X) { if (!px) continue; nx = (nx + X) % X; }
+        if (ny < 0 || ny >= Y) { if (!py) continue; ny = (ny + Y) % Y; }
+        if (nz < 0 || nz >= Z) { if (!pz) continue; nz = (nz + Z) % Z; }
+        if (ids[nx + (int64_t)X * (ny + (int64_t)Y * nz)] != 0) m |= 1u << j;
+    }
+    nmask[c] = m;
+}
+
 template <int ST, bool ODD>
 hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const LaunchShape &sh,
                           double *partial, hipStream_t stream) {
@@ -682,6 +745,15 @@ hipError_t launch_step<BF_VS>(const StepArgs &a, int storage, bool odd, bool wal
                              : launch_step_st<ST_BF16, false>(a, walls, mp, th, sh, partial, stream);
     }
     return hipErrorInvalidValue;
+}
+
+template <>
+hipError_t launch_mask<BF_VS>(const uint8_t *ids, uint32_t *nmask, int X, int Y, int Z, int px, int py, int pz,
+                              hipStream_t stream) {
+    const int64_t n = (int64_t)X * Y * Z;
+    const dim3 block(256), grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL((k_mask<BF_VS>), grid, block, 0, stream, ids, nmask, X, Y, Z, px, py, pz);
+    return hipGetLastError();
 }
 
 template <>
