@@ -19,10 +19,14 @@ struct alignas(16) ObjectEntry {
     float ux, uy, uz, temperature;
 };
 
+constexpr int kMaxQ = 27, kMaxQT = 7;
+
 struct StepArgs {
     void *f;                 // lattice, Q slots of `stride` elements (storage type)
     float *g;                // thermal lattice, QT slots of `stride` floats (or null)
     int64_t stride;          // elements between slots
+    void *slot[kMaxQ];       // base of slot i (f + i * stride)
+    float *gslot[kMaxQT];    // base of thermal slot q (g + q * stride)
     int X, Y, Z;
     int periodic_x, periodic_y, periodic_z;
     const uint8_t *ids;      // boundary id per cell (or null)

@@ -173,6 +173,9 @@ StepArgs make_args(bf_solver *s, bool update) {
     a.f = s->f;
     a.g = s->g;
     a.stride = s->stride;
+    for (int i = 0; i < s->q; ++i) a.slot[i] = static_cast<char *>(s->f) + (size_t)i * s->stride * s->elem;
+    if (s->g)
+        for (int q = 0; q < s->qt; ++q) a.gslot[q] = s->g + (size_t)q * s->stride;
     a.X = s->X; a.Y = s->Y; a.Z = s->Z;
     a.periodic_x = s->cfg.periodic[0] != 0;
     a.periodic_y = s->cfg.periodic[1] != 0;
@@ -298,8 +301,11 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
     if (!(cfg->relaxation_time > 0.5f))
         return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: relaxation time %g must exceed 0.5", (double)cfg->relaxation_time);
     const int64_t N = (int64_t)X * Y * Z;
-    if (N >= (int64_t(1) << 31))
-        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: %lld cells exceed the 2^31 cell index range", (long long)N);
+    // each slot (stride cells of up to 4 bytes) is addressed with 32-bit byte
+    // offsets; the reference's uint32 element addressing (collide.comp:196-200)
+    // allows Q * N < 2^32 only
+    if (N > (int64_t(1) << 30) - 64)
+        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: %lld cells exceed the 2^30 cell limit", (long long)N);
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
@@ -757,6 +763,9 @@ bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, u
 
 namespace bf {
 
+#ifndef BF_ALIGN_ROWS
+#define BF_ALIGN_ROWS 1  // build-time switch of the aligned even tick
+#endif
 #ifndef BF_BLOCK_CELLS
 #define BF_BLOCK_CELLS 256
 #endif
@@ -785,7 +794,7 @@ LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cpt, int vs) {
             const int bx = row_threads;
             const int by = std::max(1, BF_BLOCK_CELLS / bx);
             sh.block = dim3(bx, by, 1);
-            sh.lds_bytes = sizeof(float) * (size_t)x_direction_count(vs) * by * X;
+            sh.lds_bytes = BF_ALIGN_ROWS ? sizeof(float) * (size_t)x_direction_count(vs) * by * X : 0;
         } else {
             const int bx = std::min(BF_BLOCK_CELLS, row_threads);
             const int by = std::max(1, BF_BLOCK_CELLS / bx);

@@ -64,30 +64,43 @@ __device__ inline void st(T *p, T v) {
 #endif
 }
 
+// Element `idx` of a slot whose base is wave-uniform, addressed as base + a
+// 32-bit byte offset: the loads and stores take the scalar-base form and cells
+// that share an index share one offset register.  The host keeps every slot
+// below 4 GiB (stride * element size < 2^32).
+template <class T>
+__device__ inline const T *at(const T *p, uint32_t idx) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ inline T *at(T *p, uint32_t idx) {
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + idx * (uint32_t)sizeof(T));
+}
+
 // Storage policies: FP32 as is; FP16 and BF16 hold f - w.  `raw` issues the
 // load, `cvt` turns the stored value into f; the tick issues all raw loads of a
 // cell before it converts any, so the loads stay in flight together.
 template <int ST> struct Store;
 template <> struct Store<ST_FP32> {
     using T = float;
-    __device__ static T raw(const T *p, uint32_t idx) { return ld(p + idx); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v, float) { return v; }
     __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, uint32_t idx, float v, float) { st(p + idx, v); }
+    __device__ static void store(T *p, uint32_t idx, float v, float) { st(at(p, idx), v); }
 };
 template <> struct Store<ST_FP16> {
     using T = _Float16;
-    __device__ static T raw(const T *p, uint32_t idx) { return ld(p + idx); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v, float w) { return (float)v + w; }
     __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(p + idx, (_Float16)(v - w)); }
+    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(at(p, idx), (_Float16)(v - w)); }
 };
 template <> struct Store<ST_BF16> {
     using T = uint16_t;
-    __device__ static T raw(const T *p, uint32_t idx) { return ld(p + idx); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v, float w) { return __uint_as_float((uint32_t)v << 16) + w; }
     __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(p + idx, bf16_rne(v - w)); }
+    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(at(p, idx), bf16_rne(v - w)); }
 };
 
 // c_i . v summed over the non-zero components of c_i in x, y, z order.  Equal
@@ -159,24 +172,22 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     constexpr int QT = L::QT;
     constexpr bool D3 = L::DIM == 3;
 
-    T *__restrict__ f = static_cast<T *>(a.f);
+    // slot bases come from the argument block (wave-uniform, scalar registers)
+#define SLOT(i) static_cast<T *>(a.slot[i])
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int y = blockIdx.y * blockDim.y + ty;
     const int z = blockIdx.z;
 
     // Neighbour coordinates: index 0 = -1, 1 = 0, 2 = +1 along each axis.
-    int ys[3] = {y - 1, y, y + 1};
-    bool vy[3] = {true, true, true};
-    if (ys[0] < 0) { if (a.periodic_y) ys[0] += Y; else vy[0] = false; }
-    if (ys[2] >= Y) { if (a.periodic_y) ys[2] -= Y; else vy[2] = false; }
-    int zs[3] = {z, z, z};
-    bool vz[3] = {true, true, true};
-    if (D3) {
-        zs[0] = z - 1; zs[2] = z + 1;
-        if (zs[0] < 0) { if (a.periodic_z) zs[0] += Z; else vz[0] = false; }
-        if (zs[2] >= Z) { if (a.periodic_z) zs[2] -= Z; else vz[2] = false; }
-    }
+    // Written as selects: a branch here would split the block that issues the
+    // loads and cost the scalar-base addressing of every access.
+    const bool wy0 = y - 1 < 0, wy2 = y + 1 >= Y;
+    const int ys[3] = {wy0 ? y - 1 + Y : y - 1, y, wy2 ? y + 1 - Y : y + 1};
+    const bool vy[3] = {!wy0 || a.periodic_y != 0, true, !wy2 || a.periodic_y != 0};
+    const bool wz0 = D3 && z - 1 < 0, wz2 = D3 && z + 1 >= Z;
+    const int zs[3] = {D3 ? (wz0 ? z - 1 + Z : z - 1) : z, z, D3 ? (wz2 ? z + 1 - Z : z + 1) : z};
+    const bool vz[3] = {!wz0 || a.periodic_z != 0, true, !wz2 || a.periodic_z != 0};
     const int ry[3] = {ys[0] * X, ys[1] * X, ys[2] * X};
     const int XY = X * Y;
     const int pz[3] = {zs[0] * XY, zs[1] * XY, zs[2] * XY};
@@ -190,10 +201,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int x = (blockIdx.x * CPT + k) * blockDim.x + tx;
-        xs[k][0] = x - 1; xs[k][1] = x; xs[k][2] = x + 1;
-        vx[k][0] = vx[k][1] = vx[k][2] = true;
-        if (xs[k][0] < 0) { if (a.periodic_x) xs[k][0] += X; else vx[k][0] = false; }
-        if (xs[k][2] >= X) { if (a.periodic_x) xs[k][2] -= X; else vx[k][2] = false; }
+        const bool w0 = x - 1 < 0, w2 = x + 1 >= X;
+        xs[k][0] = w0 ? x - 1 + X : x - 1; xs[k][1] = x; xs[k][2] = w2 ? x + 1 - X : x + 1;
+        vx[k][0] = !w0 || a.periodic_x != 0; vx[k][1] = true; vx[k][2] = !w2 || a.periodic_x != 0;
         self[k] = x + ry[1] + pz[1];
         active[k] = x < X && y < Y;
     }
@@ -262,7 +272,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 if (L::c[i][0] == 0) continue;
                 const int dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
                 const int idx = !in_grid ? 0 : (PER || (vy[dy] && vz[dz])) ? xs[k][1] + ry[dy] + pz[dz] : self[k];
-                raw[k][i] = S::raw(f + (size_t)i * a.stride, idx);
+                raw[k][i] = S::raw(SLOT(i), idx);
             }
         }
         const bool act = active[k];
@@ -270,7 +280,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         for (int i = 0; i < Q; ++i) {
             const int o = opposite(i);
             if (ODD || i == 0) {
-                raw[k][i] = S::raw(f + (size_t)(ODD ? o : 0) * a.stride, act ? self[k] : 0);
+                raw[k][i] = S::raw(SLOT((ODD ? o : 0)), act ? self[k] : 0);
                 continue;
             }
             const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
@@ -280,7 +290,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 // own opposite slot for a wall redirect: loaded below
             } else {
                 const int idx = !act ? 0 : (valid && !redir) ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
-                T *const pi = f + (size_t)i * a.stride, *const po = f + (size_t)o * a.stride;
+                T *const pi = SLOT(i), *const po = SLOT(o);
                 raw[k][i] = S::raw(redir ? po : pi, idx);
             }
         }
@@ -289,13 +299,13 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int q = 0; q < QT; ++q) {
                 const int o = opposite(q);
                 if (ODD || q == 0) {
-                    graw[q] = a.g[(size_t)(ODD ? o : 0) * a.stride + (act ? self[k] : 0)];
+                    graw[q] = *at(a.gslot[(ODD ? o : 0)], act ? self[k] : 0);
                     continue;
                 }
                 const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
                 const bool pull = OK(k, dx, dy, dz) && !SOLID(k, o);
                 const int idx = !act ? 0 : pull ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
-                graw[q] = a.g[(size_t)(pull ? q : o) * a.stride + idx];
+                graw[q] = *at(pull ? a.gslot[q] : a.gslot[o], idx);
             }
         }
     }
@@ -396,7 +406,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 for (int i = 1; i < Q; ++i) {
                     if (L::c[i][0] == 0) continue;
                     const int o = opposite(i);
-                    rr[i] = S::raw(f + (size_t)o * a.stride, (active[k] && a.halfway && SOLID(k, o)) ? self[k] : 0);
+                    rr[i] = S::raw(SLOT(o), (active[k] && a.halfway && SOLID(k, o)) ? self[k] : 0);
                 }
 #pragma unroll
                 for (int i = 1; i < Q; ++i) {
@@ -508,7 +518,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     }
                 }
                 if (ODD || i == 0) {
-                    S::store(f + (size_t)i * a.stride, self[k], v, w);
+                    S::store(SLOT(i), self[k], v, w);
                 } else {
                     const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                     if (!OK(k, dx, dy, dz)) {
@@ -519,11 +529,11 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                             const ObjectEntry ob = WALL(k, i);
                             vb = __builtin_fmaf(-6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
                         }
-                        S::store(f + (size_t)i * a.stride, self[k], vb, w);
+                        S::store(SLOT(i), self[k], vb, w);
                     } else if (AL && L::c[i][0] != 0) {
                         LDS(i, xs[k][dx]) = v;
                     } else {
-                        S::store(f + (size_t)o * a.stride, xs[k][dx] + ry[dy] + pz[dz], v, w);
+                        S::store(SLOT(o), xs[k][dx] + ry[dy] + pz[dz], v, w);
                     }
                 }
             }
@@ -548,16 +558,16 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     const float e = (L::wt[q] * temp) * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
                     const float v = __builtin_fmaf(e - gi[q], inv_tau_t, gi[q]);
                     if (ODD || q == 0) {
-                        a.g[(size_t)q * a.stride + self[k]] = v;
+                        *at(a.gslot[q], self[k]) = v;
                     } else {
                         const int dx = 1 + L::c[q][0], dy = 1 + L::c[q][1], dz = 1 + L::c[q][2];
                         if (!OK(k, dx, dy, dz)) {
-                            a.g[(size_t)q * a.stride + self[k]] = v;
+                            *at(a.gslot[q], self[k]) = v;
                         } else if (SOLID(k, q)) {
-                            a.g[(size_t)q * a.stride + self[k]] =
+                            *at(a.gslot[q], self[k]) =
                                 __builtin_fmaf(2.0f * L::wt[q], WALL(k, q).temperature, -v);
                         } else {
-                            a.g[(size_t)o * a.stride + xs[k][dx] + ry[dy] + pz[dz]] = v;
+                            *at(a.gslot[o], xs[k][dx] + ry[dy] + pz[dz]) = v;
                         }
                     }
                 }
@@ -584,12 +594,13 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 const float v = LDS(i, x);
                 if (__float_as_uint(v) == kNoValue) continue;
                 const int dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
-                S::store(f + (size_t)opposite(i) * a.stride, x + ry[dy] + pz[dz], v, L::w[i]);
+                S::store(SLOT(opposite(i)), x + ry[dy] + pz[dz], v, L::w[i]);
             }
         }
     }
 
 #undef LDS
+#undef SLOT
 #undef OK
 #undef SOLID
 #undef WALL
