@@ -7,6 +7,7 @@ tick over the whole grid; the populations stay resident in HBM.
 Prints one JSON line:
   value / roofline        MLUPs/s of the FP32 run; HBM roofline of the tick kernel
                           (algorithmic 152 B/cell / device-timed launch duration)
+  ceiling                 copy and in-place 19-slot streaming probes (achievable GB/s)
   fp16_shifted, bf16_...  the same run with 16-bit shifted storage
   error_vs_fp32           16-bit fields vs the FP32 run (same workload at 128^3)
   other_configs           porous 384^3 D3Q19 (FP32/FP16/BF16, + error at 128^3),
@@ -288,12 +289,22 @@ def main():
                    "cells": n ** 3, "bytes_per_cell": bytes_per_cell(19, "fp32")},
         "roofline": main_run["roofline"],
     }
+    # achievable-bandwidth ceilings on this device: a plain copy, and the tick's
+    # own access pattern (19 FP32 slots read and rewritten in place) without
+    # arithmetic, at the FP32 lattice size and at the 16-bit lattice's bytes
+    copy = bfa.capi.copy_bandwidth(1 << 30, 20)
+    inplace = bfa.capi.inplace_bandwidth(n ** 3, 19, 20)
+    inplace_half = bfa.capi.inplace_bandwidth(n ** 3 // 2, 19, 20)
+    line["ceiling"] = {"copy_GBs": round(copy, 1), "inplace_19slot_GBs": round(inplace, 1),
+                       "inplace_19slot_half_bytes_GBs": round(inplace_half, 1),
+                       "fp32_kernel_frac_of_inplace": round(main_run["roofline"]["achieved"] / inplace, 4)}
     for st in ("fp16", "bf16"):
         if st in res:
             r = res[st]
             line[f"{st}_shifted"] = {"value": round(r["mlups"], 1), "unit": "MLUPs/s",
                                      "speedup_vs_fp32": round(r["mlups"] / main_run["mlups"], 3),
-                                     "roofline": r["roofline"]}
+                                     "roofline": r["roofline"],
+                                     "frac_of_inplace_half_bytes": round(r["roofline"]["achieved"] / inplace_half, 4)}
     if not args.no_accuracy and len(res) > 1:
         line["error_vs_fp32"] = accuracy_vs_fp32(bfa, list(res))
     if not args.no_other_configs:

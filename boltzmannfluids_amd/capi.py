@@ -109,6 +109,7 @@ def load_library(path=None):
         "bf_pack_boundaries": (i32, [C.POINTER(C.c_uint8), i64, i32, C.POINTER(C.c_uint32)]),
         "bf_unpack_boundaries": (i32, [C.POINTER(C.c_uint32), i64, i32, C.POINTER(C.c_uint8)]),
         "bf_probe_copy_bandwidth": (i32, [i32, i64, i32, C.POINTER(C.c_double)]),
+        "bf_probe_inplace_bandwidth": (i32, [i32, i64, i32, i32, C.POINTER(C.c_double)]),
         "bf_channel_create": (i32, [C.POINTER(ChannelConfig), C.POINTER(vp)]),
         "bf_channel_destroy": (None, [vp]),
         "bf_channel_last_error": (C.c_char_p, [vp]),
@@ -186,6 +187,18 @@ def copy_bandwidth(nbytes=2 << 30, iters=20, device=0):
     st = lib.bf_probe_copy_bandwidth(int(device), int(nbytes), int(iters), C.byref(out))
     if st:
         raise BFError(st, "bf_probe_copy_bandwidth")
+    return out.value
+
+
+def inplace_bandwidth(cells, slots=19, iters=20, device=0):
+    """GB/s (read + write) of the tick's access pattern without arithmetic:
+    `slots` FP32 structure-of-arrays slots of `cells` elements read and
+    rewritten in place."""
+    lib = load_library()
+    out = C.c_double()
+    st = lib.bf_probe_inplace_bandwidth(int(device), int(cells), int(slots), int(iters), C.byref(out))
+    if st:
+        raise BFError(st, "bf_probe_inplace_bandwidth")
     return out.value
 
 

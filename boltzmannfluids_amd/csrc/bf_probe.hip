@@ -1,33 +1,82 @@
-// bf_probe.hip — device copy-bandwidth probe.
+// bf_probe.hip — device bandwidth probes.
 //
-// Streams `bytes` from one device buffer to another with 16-byte accesses and
-// reports the sustained rate (read + write bytes per second).  It gives the
-// achievable HBM bandwidth the tick kernel is compared against, and a kernel of
-// known traffic for calibrating the FETCH_SIZE / WRITE_SIZE counters.
+// bf_probe_copy_bandwidth streams `bytes` from one device buffer to another
+// with 16-byte accesses and reports the sustained rate (read + write bytes per
+// second): the achievable HBM bandwidth of a plain copy, and a kernel of known
+// traffic for calibrating the FETCH_SIZE / WRITE_SIZE counters.
+//
+// bf_probe_inplace_bandwidth reproduces the memory traffic of one lattice tick
+// without its arithmetic: every thread reads one 4-byte element of each of
+// `slots` arrays of `cells` elements (structure-of-arrays, slot stride rounded
+// to 64 cells like the solver's lattice) and writes a value back to the same
+// places.  Its rate is the ceiling of the in-place tick's access pattern.
 #include <hip/hip_runtime.h>
-
 
 #include "../../include/bf/bf.h"
 
 namespace {
 
-// Each block copies 4 x 256 consecutive float4 (16 KiB); all four loads are
-// issued before the stores, with streaming (nontemporal) hints.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// one float4 per thread, streaming (nontemporal) hints
 __global__ void __launch_bounds__(256) k_copy(const v4f *__restrict__ src, v4f *__restrict__ dst, size_t n) {
-    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
-    v4f v[4];
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+constexpr int kMaxSlots = 27;
+
+// all loads of a thread are issued before its stores, as in the tick kernel
+template <int NS>
+__global__ void __launch_bounds__(256) k_inplace(float *__restrict__ f, int64_t stride, int64_t cells) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= cells) return;
+    float v[NS];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const size_t i = base + (size_t)k * 256;
-        if (i < n) v[k] = __builtin_nontemporal_load(src + i);
-    }
+    for (int s = 0; s < NS; ++s) v[s] = __builtin_nontemporal_load(f + s * stride + c);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const size_t i = base + (size_t)k * 256;
-        if (i < n) __builtin_nontemporal_store(v[k], dst + i);
+    for (int s = 0; s < NS; ++s) __builtin_nontemporal_store(0.5f * v[s] + v[(s + 1) % NS], f + s * stride + c);
+}
+
+template <int NS>
+void launch_inplace(float *f, int64_t stride, int64_t cells, hipStream_t s) {
+    hipLaunchKernelGGL((k_inplace<NS>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, f, stride, cells);
+}
+
+void launch_inplace_n(int slots, float *f, int64_t stride, int64_t cells, hipStream_t s) {
+    switch (slots) {
+    case 5: launch_inplace<5>(f, stride, cells, s); break;
+    case 7: launch_inplace<7>(f, stride, cells, s); break;
+    case 9: launch_inplace<9>(f, stride, cells, s); break;
+    case 15: launch_inplace<15>(f, stride, cells, s); break;
+    case 19: launch_inplace<19>(f, stride, cells, s); break;
+    case 27: launch_inplace<27>(f, stride, cells, s); break;
+    default: break;
     }
+}
+
+// Times `iters` launches of `launch` after one warm-up launch on a private stream.
+template <class F>
+bf_status time_launches(F launch, int32_t iters, float *ms) {
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bf_status st = BF_OK;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        st = BF_ERR_DEVICE;
+    } else {
+        launch(s);
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < iters; ++k) launch(s);
+        (void)hipEventRecord(e1, s);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+            st = BF_ERR_DEVICE;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    return st;
 }
 
 }  // namespace
@@ -39,28 +88,37 @@ extern "C" bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int3
     float4 *a = nullptr, *b = nullptr;
     if (hipMalloc((void **)&a, n * sizeof(float4)) != hipSuccess) return BF_ERR_OOM;
     if (hipMalloc((void **)&b, n * sizeof(float4)) != hipSuccess) { (void)hipFree(a); return BF_ERR_OOM; }
-    hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    bf_status st = BF_OK;
     float ms = 0.0f;
-    const unsigned grid = (unsigned)((n + 1023) / 1024);
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess) {
-        st = BF_ERR_DEVICE;
-    } else {
-        (void)hipMemsetAsync(a, 0, n * sizeof(float4), s);
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, (const v4f *)a, (v4f *)b, n);  // warm-up
-        (void)hipEventRecord(e0, s);
-        for (int k = 0; k < iters; ++k)
+    bf_status st = BF_OK;
+    if (hipMemset(a, 0, n * sizeof(float4)) != hipSuccess) st = BF_ERR_DEVICE;
+    if (st == BF_OK) {
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        st = time_launches([&](hipStream_t s) {
             hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, (const v4f *)a, (v4f *)b, n);
-        (void)hipEventRecord(e1, s);
-        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = BF_ERR_DEVICE;
+        }, iters, &ms);
     }
     if (st == BF_OK) *gbps = 2.0 * (double)(n * sizeof(float4)) * iters / (ms * 1e-3) / 1e9;
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (s) (void)hipStreamDestroy(s);
     (void)hipFree(a);
     (void)hipFree(b);
+    return st;
+}
+
+extern "C" bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, int32_t slots, int32_t iters,
+                                                double *gbps) {
+    if (!gbps || cells < 64 || iters < 1) return BF_ERR_INVALID;
+    if (!(slots == 5 || slots == 7 || slots == 9 || slots == 15 || slots == 19 || slots == kMaxSlots))
+        return BF_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return BF_ERR_DEVICE;
+    const int64_t stride = (cells + 63) / 64 * 64;
+    const size_t bytes = sizeof(float) * (size_t)stride * slots;
+    float *f = nullptr;
+    if (hipMalloc((void **)&f, bytes) != hipSuccess) return BF_ERR_OOM;
+    float ms = 0.0f;
+    bf_status st = BF_OK;
+    if (hipMemset(f, 0, bytes) != hipSuccess) st = BF_ERR_DEVICE;
+    if (st == BF_OK)
+        st = time_launches([&](hipStream_t s) { launch_inplace_n(slots, f, stride, cells, s); }, iters, &ms);
+    if (st == BF_OK) *gbps = 2.0 * sizeof(float) * (double)cells * slots * iters / (ms * 1e-3) / 1e9;
+    (void)hipFree(f);
     return st;
 }

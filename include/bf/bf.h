@@ -300,6 +300,13 @@ bf_status bf_channel_timing(const bf_channel *c, double out[3]);
  * published 8 TB/s, and the known-traffic kernel for counter calibration. */
 bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t iters, double *gbps);
 
+/* In-place streaming probe (no counterpart in the reference): the memory
+ * traffic of one lattice tick without its arithmetic — every cell reads and
+ * rewrites one FP32 element in each of `slots` (5, 7, 9, 15, 19 or 27)
+ * structure-of-arrays slots of `cells` elements.  *gbps = read + write bytes
+ * per second over `iters` launches: the ceiling of the tick's access pattern. */
+bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, int32_t slots, int32_t iters, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif
