@@ -67,6 +67,10 @@ __device__ constexpr float zh_wt(int i) {
     return nz == 1 ? (L::Q == 9 ? 2.0f / 3.0f : 1.0f / 3.0f) : 1.0f / 6.0f;
 }
 
+// Every state and adjoint slot is read once and written once per tick: streaming hints.
+__device__ inline float ldnt(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ inline void stnt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
 struct CellPos {
     int x, y, z, self;
     int zm, zp;   // periodic z neighbours
@@ -82,10 +86,10 @@ __device__ inline void pull(const ChArgs &a, const float *__restrict__ src, cons
         const int sz = L::c[i][2] > 0 ? p.zm : (L::c[i][2] < 0 ? p.zp : p.z);
         if (L::c[i][0] != 0 && (sx < 0 || sx >= a.X)) { f[i] = 0.0f; continue; }
         if (L::c[i][1] != 0 && (sy == 0 || sy == a.Y - 1)) {
-            f[i] = src[(size_t)opposite(i) * a.stride + p.self];
+            f[i] = ldnt(src + (size_t)opposite(i) * a.stride + p.self);
             continue;
         }
-        f[i] = src[(size_t)i * a.stride + sx + sy * a.X + sz * XY];
+        f[i] = ldnt(src + (size_t)i * a.stride + sx + sy * a.X + sz * XY);
     }
 }
 
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(256) k_ch_step(const ChArgs a, const float *__
     }
 #pragma unroll
     for (int i = 0; i < L::Q; ++i)
-        dst[(size_t)i * a.stride + p.self] = (1.0f - g) * fs[i] + g * f[opposite(i)];
+        stnt(dst + (size_t)i * a.stride + p.self, (1.0f - g) * fs[i] + g * f[opposite(i)]);
 }
 
 template <int VS>
@@ -189,6 +193,12 @@ __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float 
     constexpr int Q = L::Q;
     CellPos p;
     if (!cell_of<VS>(a, p)) return;
+    // the adjoint slots are loaded before the state is pulled: both sets of
+    // loads are in flight together
+    float mu[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) mu[i] = ldnt(nu + (size_t)i * a.stride + p.self);
+    const float g = a.gamma[p.self];
     float f[Q];
     int sign;
     float S, ny, nz, uin;
@@ -207,10 +217,6 @@ __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float 
     }
     const float ux = mx / rho, uy = my / rho, uz = mz / rho;
     const float uu = ux * ux + uy * uy + uz * uz;
-    const float g = a.gamma[p.self];
-    float mu[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) mu[i] = nu[(size_t)i * a.stride + p.self];
 
     // d post_i / d gamma = f_opp(i) - f*_i; transpose of collision + partial bounce-back
     float acc = 0.0f, A = 0.0f, C = 0.0f, Bx = 0.0f, By = 0.0f, Bz = 0.0f;
@@ -272,10 +278,10 @@ __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float 
         const int sz = L::c[i][2] > 0 ? p.zm : (L::c[i][2] < 0 ? p.zp : p.z);
         if (L::c[i][0] != 0 && (sx < 0 || sx >= a.X)) continue;
         if (L::c[i][1] != 0 && (sy == 0 || sy == a.Y - 1)) {
-            nu_prev[(size_t)opposite(i) * a.stride + p.self] = mh[i];
+            stnt(nu_prev + (size_t)opposite(i) * a.stride + p.self, mh[i]);
             continue;
         }
-        nu_prev[(size_t)i * a.stride + sx + sy * a.X + sz * XY] = mh[i];
+        stnt(nu_prev + (size_t)i * a.stride + sx + sy * a.X + sz * XY, mh[i]);
     }
 }
 
