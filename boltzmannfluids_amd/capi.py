@@ -110,6 +110,13 @@ def load_library(path=None):
         "bf_unpack_boundaries": (i32, [C.POINTER(C.c_uint32), i64, i32, C.POINTER(C.c_uint8)]),
         "bf_probe_copy_bandwidth": (i32, [i32, i64, i32, C.POINTER(C.c_double)]),
         "bf_probe_inplace_bandwidth": (i32, [i32, i64, i32, i32, C.POINTER(C.c_double)]),
+        "bf_marching_cubes_capacity": (i32, [C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "bf_marching_cubes": (i32, [i32, f32p, C.POINTER(C.c_int32), C.c_float, f32p, f32p, i64,
+                                    C.POINTER(C.c_uint32), i64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "bf_isosurface": (i32, [vp, C.c_float, f32p, f32p, i64, C.POINTER(C.c_uint32), i64,
+                                C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "bf_marching_cubes_table": (i32, [C.POINTER(C.c_uint8), C.POINTER(C.c_int8)]),
+        "bf_marching_cubes_last_error": (C.c_char_p, []),
         "bf_channel_create": (i32, [C.POINTER(ChannelConfig), C.POINTER(vp)]),
         "bf_channel_destroy": (None, [vp]),
         "bf_channel_last_error": (C.c_char_p, [vp]),
@@ -200,6 +207,46 @@ def inplace_bandwidth(cells, slots=19, iters=20, device=0):
     if st:
         raise BFError(st, "bf_probe_inplace_bandwidth")
     return out.value
+
+
+def marching_cubes_table():
+    """The case table: (triangle_count[256] uint8, edges[256, 15] int8)."""
+    lib = load_library()
+    n = np.zeros(256, np.uint8)
+    e = np.zeros((256, 15), np.int8)
+    st = lib.bf_marching_cubes_table(n.ctypes.data_as(C.POINTER(C.c_uint8)), e.ctypes.data_as(C.POINTER(C.c_int8)))
+    if st:
+        raise BFError(st, "bf_marching_cubes_table")
+    return n, e
+
+
+def _mc_run(call):
+    """Size query, then extraction into arrays of the returned sizes."""
+    lib = load_library()
+    nv, ni = C.c_int64(), C.c_int64()
+    st = call(None, 0, None, 0, C.byref(nv), C.byref(ni))
+    if st:
+        raise BFError(st, lib.bf_marching_cubes_last_error().decode())
+    verts = np.zeros((max(nv.value, 1), 3), np.float32)
+    idx = np.zeros((max(ni.value, 3) // 3, 3), np.uint32)
+    st = call(_fptr(verts), nv.value, idx.ctypes.data_as(C.POINTER(C.c_uint32)), ni.value, C.byref(nv), C.byref(ni))
+    if st:
+        raise BFError(st, lib.bf_marching_cubes_last_error().decode())
+    return verts[: nv.value], idx[: ni.value // 3]
+
+
+def marching_cubes(volume, threshold, mesh_scale=(1.0, 1.0, 1.0), device=0):
+    """Iso-surface of a scalar field given as an array of shape (Z, Y, X) (or
+    flat with `volume.shape` set by the caller): returns vertices (V, 3) float32
+    and triangles (T, 3) uint32."""
+    lib = load_library()
+    v = np.ascontiguousarray(volume, np.float32)
+    if v.ndim != 3:
+        raise ValueError("volume must have shape (Z, Y, X)")
+    res = (C.c_int32 * 3)(v.shape[2], v.shape[1], v.shape[0])
+    sc = np.ascontiguousarray(mesh_scale, np.float32)
+    return _mc_run(lambda vp_, vc, ip, ic, nv, ni: lib.bf_marching_cubes(
+        int(device), _fptr(v), res, float(threshold), _fptr(sc), vp_, vc, ip, ic, nv, ni))
 
 
 class Solver:
@@ -304,6 +351,13 @@ class Solver:
         i = Info()
         self._check(self._lib.bf_get_info(self._h, C.byref(i)), "bf_get_info")
         return {k: (list(getattr(i, k)) if k == "resolution" else getattr(i, k)) for k, _ in Info._fields_}
+
+    def isosurface(self, threshold, mesh_scale=(1.0, 1.0, 1.0)):
+        """Marching-cubes iso-surface of the density saved on the last save
+        tick, extracted on the device: vertices (V, 3) and triangles (T, 3)."""
+        sc = np.ascontiguousarray(mesh_scale, np.float32)
+        return _mc_run(lambda vp_, vc, ip, ic, nv, ni: self._lib.bf_isosurface(
+            self._h, float(threshold), _fptr(sc), vp_, vc, ip, ic, nv, ni))
 
     def read(self, field):
         fid = FIELDS.get(field, field)

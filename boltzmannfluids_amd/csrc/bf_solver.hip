@@ -631,6 +631,18 @@ size_t bf_field_bytes(const bf_solver *s, int32_t field) {
     return 0;
 }
 
+// Device view of the saved density (the w channel of the velocity_density
+// field) for the iso-surface pass (bf_mc.hip).
+bf_status bf_solver_density_field(bf_solver *s, const float **v, int *step, int32_t res[3], hipStream_t *stream) {
+    if (!s || !s->initialized || !s->vd) return BF_ERR_STATE;
+    if (hipSetDevice(s->cfg.device) != hipSuccess) return BF_ERR_DEVICE;
+    *v = reinterpret_cast<const float *>(s->vd) + 3;
+    *step = 4;
+    res[0] = s->X; res[1] = s->Y; res[2] = s->Z;
+    *stream = s->stream;
+    return BF_OK;
+}
+
 bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes) {
     if (!s || !dst) return BF_ERR_INVALID;
     const size_t need = bf_field_bytes(s, field);

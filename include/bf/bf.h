@@ -293,6 +293,44 @@ bf_status bf_channel_read_moments(bf_channel *c, float *out, size_t bytes);
  * [1] recomputation, [2] whole gradient (0 for a forward-only run). */
 bf_status bf_channel_timing(const bf_channel *c, double out[3]);
 
+/* --- iso-surface (marching cubes) ---------------------------------------
+ * Replaces MarchingCubes::compute (MarchingCubes/MarchingCubes.cpp:3-48,
+ * GLSL/MarchingCubes/marching_cubes.comp:41-98), which the reference never
+ * dispatches.  Output is an indexed triangle mesh:
+ *   vertices — 3 floats each, one per grid edge whose end values straddle the
+ *              threshold, ordered by (grid point, axis x/y/z), linearly
+ *              interpolated and placed at (p / resolution - 1/2) * mesh_scale
+ *              (the reference's vertex placement, marching_cubes.comp:67-70);
+ *   indices  — 3 uint32 per triangle, ordered by (cube, triangle of its case);
+ *              points with value >= threshold are inside, and the winding gives
+ *              normals toward lower values.
+ * Passing NULL for both output arrays only returns the counts.  Too small a
+ * capacity returns BF_ERR_INVALID with the needed counts set.  The text of the
+ * last failure is in bf_marching_cubes_last_error(). */
+
+/* Worst-case output sizes for a grid (get_vertex_buffer_needed_size_in_bytes /
+ * get_index_buffer_needed_size_in_bytes, MarchingCubes.h:17-18). */
+bf_status bf_marching_cubes_capacity(const int32_t resolution[3], int64_t *max_vertices, int64_t *max_indices);
+
+/* Iso-surface of a host scalar field (X*Y*Z floats in cell order), on `device`. */
+bf_status bf_marching_cubes(int32_t device, const float *volume, const int32_t resolution[3], float threshold,
+                            const float mesh_scale[3], float *vertices, int64_t vertex_capacity, uint32_t *indices,
+                            int64_t index_capacity, int64_t *vertex_count, int64_t *index_count);
+
+/* Iso-surface of a solver's density as of its last save tick, computed on the
+ * device from the resident field (no host copy of the volume). */
+bf_status bf_isosurface(bf_solver *solver, float threshold, const float mesh_scale[3], float *vertices,
+                        int64_t vertex_capacity, uint32_t *indices, int64_t index_capacity, int64_t *vertex_count,
+                        int64_t *index_count);
+
+/* The case table in use: triangle_count[256] and edges[256 * 15] (edge ids of
+ * each triangle's corners, -1 padded).  Corner k of a cube sits at
+ * (k & 1, k >> 1 & 1, k >> 2 & 1); edge 4a + j runs along axis a from the j-th
+ * corner without bit a.  The case index has bit k set when corner k is inside. */
+bf_status bf_marching_cubes_table(uint8_t *triangle_count, int8_t *edges);
+
+const char *bf_marching_cubes_last_error(void);
+
 /* --- diagnostics (no counterpart in the reference) ---------------------- */
 
 /* Device copy bandwidth: copies `bytes` between two device buffers `iters`
