@@ -432,7 +432,7 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
     *vertex_count = nv;
     *index_count = 3 * (int64_t)nt;
     if (!vertices && !indices) return BF_OK;  // size query
-    if (!vertices || !indices || vertex_capacity < (int64_t)nv || index_capacity < 3 * (int64_t)nt)
+    if ((nv && (!vertices || vertex_capacity < (int64_t)nv)) || (nt && (!indices || index_capacity < 3 * (int64_t)nt)))
         return mc_fail(BF_ERR_INVALID,
                        "[MarchingCubes Error] output too small: %lld vertices and %lld indices needed, capacity %lld / %lld",
                        (long long)nv, 3LL * nt, (long long)vertex_capacity, (long long)index_capacity);
@@ -450,8 +450,8 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
                            flags, vfirst, iout);
         MC_HIP(hipGetLastError());
     }
-    MC_HIP(hipMemcpyAsync(vertices, vout, sizeof(float) * 3 * (size_t)nv, hipMemcpyDeviceToHost, s));
-    MC_HIP(hipMemcpyAsync(indices, iout, sizeof(uint32_t) * 3 * (size_t)nt, hipMemcpyDeviceToHost, s));
+    if (nv) MC_HIP(hipMemcpyAsync(vertices, vout, sizeof(float) * 3 * (size_t)nv, hipMemcpyDeviceToHost, s));
+    if (nt) MC_HIP(hipMemcpyAsync(indices, iout, sizeof(uint32_t) * 3 * (size_t)nt, hipMemcpyDeviceToHost, s));
     MC_HIP(hipStreamSynchronize(s));
     return BF_OK;
 }

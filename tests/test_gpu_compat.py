@@ -37,3 +37,34 @@ def test_lid_cavity_through_lbm_shim(orc, tmp_path):
     print(f"  max|shim - oracle| = {d:.3e}", flush=True)
     assert d <= 1e-6
     np.testing.assert_allclose(sums[0], o.conservation()[0], rtol=1e-12)
+
+
+DROPLET = os.path.join(HERE, "compat", "droplet_isosurface_demo")
+
+
+def test_droplet_isosurface_through_marching_cubes_shim(tmp_path):
+    """LBM.h + MarchingCubes.h: the mesh from the device-resident density and the
+    one from the host copy both equal the oracle's mesh of that density."""
+    import mc_oracle as mc
+    assert os.path.exists(DROPLET), "build() compiles tests/compat/droplet_isosurface_demo"
+    out = tmp_path / "droplet.bin"
+    print("case: LBM.h Shan-Chen droplet 32^3 D3Q19, 100 ticks, MarchingCubes.h", flush=True)
+    r = subprocess.run([DROPLET, str(out), "100"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    n = 32
+    rho = np.frombuffer(raw[:4 * n ** 3], np.float32).reshape(n, n, n)
+    off = 4 * n ** 3
+    meshes = []
+    for _ in range(2):
+        nv, ni = np.frombuffer(raw[off:off + 16], np.int64)
+        off += 16
+        v = np.frombuffer(raw[off:off + 12 * nv], np.float32).reshape(-1, 3)
+        off += 12 * nv
+        t = np.frombuffer(raw[off:off + 4 * ni], np.uint32).reshape(-1, 3)
+        off += 4 * ni
+        meshes.append((v, t))
+    ov, ot = mc.marching_cubes(rho, 1.0, (n, n, n))
+    for v, t in meshes:
+        assert np.array_equal(v.view(np.uint32), ov.view(np.uint32)) and np.array_equal(t, ot)
+    assert len(ot) > 500
