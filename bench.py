@@ -197,6 +197,28 @@ def cavity_config(bfa, n=256, steps=1000, warmup=50):
             "kernel_us_per_tick": round(ev * 1e3 / steps, 2), "mlups": round(cells * steps / wall / 1e6, 1)}
 
 
+def isosurface_config(bfa, n=256, repeats=5):
+    """Marching cubes on the device of the saved density of the D3Q19 n^3 TGV
+    after 100 ticks, at the level rho = 1 (the surface p' = 0); device time of
+    the classification, prefix-sum and emit passes (best of `repeats`)."""
+    rho, u = taylor_green(n)
+    s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19")
+    s.init_fields(rho, u)
+    s.step(100, save=True)
+    best = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        v, t = s.isosurface(1.0)
+        wall = time.perf_counter() - t0
+        dev = bfa.capi.marching_cubes_device_ms()
+        best = dev if best is None else min(best, dev)
+    s.close()
+    return {"workload": f"marching cubes of the D3Q19 {n}^3 TGV density (100 ticks) at rho = 1",
+            "vertices": int(len(v)), "triangles": int(len(t)), "device_ms": round(best, 3),
+            "wall_ms_with_mesh_download": round(wall * 1e3, 2),
+            "mcells_per_s": round(n ** 3 / best / 1e3, 1)}
+
+
 def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
     """Gradient (forward with checkpoints + recomputation + adjoint) time over
     the plain forward time of the differentiable channel, device-timed.
@@ -323,7 +345,8 @@ def main():
     if not args.no_accuracy and len(res) > 1:
         line["error_vs_fp32"] = accuracy_vs_fp32(bfa, list(res))
     if not args.no_other_configs:
-        line["other_configs"] = [porous_config(bfa), spinodal_thermal_config(bfa), cavity_config(bfa)]
+        line["other_configs"] = [porous_config(bfa), spinodal_thermal_config(bfa), cavity_config(bfa),
+                                 isosurface_config(bfa, n)]
     if not args.no_adjoint:
         # checkpoints only (every 100 ticks, segments recomputed), then with as
         # much of the trajectory resident in HBM as the device holds

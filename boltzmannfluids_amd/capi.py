@@ -117,6 +117,7 @@ def load_library(path=None):
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "bf_marching_cubes_table": (i32, [C.POINTER(C.c_uint8), C.POINTER(C.c_int8)]),
         "bf_marching_cubes_last_error": (C.c_char_p, []),
+        "bf_marching_cubes_last_device_ms": (C.c_double, []),
         "bf_channel_create": (i32, [C.POINTER(ChannelConfig), C.POINTER(vp)]),
         "bf_channel_destroy": (None, [vp]),
         "bf_channel_last_error": (C.c_char_p, [vp]),
@@ -233,6 +234,11 @@ def _mc_run(call):
     if st:
         raise BFError(st, lib.bf_marching_cubes_last_error().decode())
     return verts[: nv.value], idx[: ni.value // 3]
+
+
+def marching_cubes_device_ms():
+    """Device time of the last marching-cubes extraction on this thread (ms)."""
+    return load_library().bf_marching_cubes_last_device_ms()
 
 
 def marching_cubes(volume, threshold, mesh_scale=(1.0, 1.0, 1.0), device=0):

@@ -357,6 +357,18 @@ hipError_t scan(const uint32_t *in, int64_t n, uint32_t *out, uint32_t *sums, hi
 }
 
 thread_local std::string g_mc_error;
+thread_local double g_mc_device_ms = 0.0;  // device time of the last extraction's passes
+
+struct Events {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    Events() {
+        for (auto &x : e) (void)hipEventCreate(&x);
+    }
+    ~Events() {
+        for (auto &x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
 
 bf_status mc_fail(bf_status st, const char *fmt, ...) {
     char buf[512];
@@ -413,6 +425,9 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
     MC_HIP(mem.alloc(&tfirst, sizeof(uint32_t) * (size_t)nc));
     MC_HIP(mem.alloc(&sums, sizeof(uint32_t) * (size_t)(2 * nbmax + 2)));
     const Vol g{v, step, X, Y, Z, threshold};
+    Events ev;
+    g_mc_device_ms = 0.0;
+    MC_HIP(hipEventRecord(ev.e[0], s));
     hipLaunchKernelGGL(k_mc_points, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, flags, vcount);
     MC_HIP(hipGetLastError());
     MC_HIP(scan(vcount, np, vfirst, sums, s));
@@ -425,12 +440,16 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
         MC_HIP(scan(tcount, nc, tfirst, sums_c, s));
         nbc = (nc + kScanItems - 1) / kScanItems;
     }
+    MC_HIP(hipEventRecord(ev.e[1], s));
     uint32_t nv = 0, nt = 0;
     MC_HIP(hipMemcpyAsync(&nv, sums + nbp, sizeof nv, hipMemcpyDeviceToHost, s));
     if (nc > 0) MC_HIP(hipMemcpyAsync(&nt, sums_c + nbc, sizeof nt, hipMemcpyDeviceToHost, s));
     MC_HIP(hipStreamSynchronize(s));
     *vertex_count = nv;
     *index_count = 3 * (int64_t)nt;
+    float ms = 0.0f;
+    MC_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+    g_mc_device_ms = ms;
     if (!vertices && !indices) return BF_OK;  // size query
     if ((nv && (!vertices || vertex_capacity < (int64_t)nv)) || (nt && (!indices || index_capacity < 3 * (int64_t)nt)))
         return mc_fail(BF_ERR_INVALID,
@@ -440,6 +459,7 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
     uint32_t *iout = nullptr;
     MC_HIP(mem.alloc(&vout, sizeof(float) * 3 * (size_t)nv));
     MC_HIP(mem.alloc(&iout, sizeof(uint32_t) * 3 * (size_t)nt));
+    MC_HIP(hipEventRecord(ev.e[2], s));
     if (nv) {
         hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, flags, vfirst,
                            mesh_scale[0], mesh_scale[1], mesh_scale[2], vout);
@@ -450,9 +470,12 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
                            flags, vfirst, iout);
         MC_HIP(hipGetLastError());
     }
+    MC_HIP(hipEventRecord(ev.e[3], s));
     if (nv) MC_HIP(hipMemcpyAsync(vertices, vout, sizeof(float) * 3 * (size_t)nv, hipMemcpyDeviceToHost, s));
     if (nt) MC_HIP(hipMemcpyAsync(indices, iout, sizeof(uint32_t) * 3 * (size_t)nt, hipMemcpyDeviceToHost, s));
     MC_HIP(hipStreamSynchronize(s));
+    MC_HIP(hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
+    g_mc_device_ms += ms;
     return BF_OK;
 }
 
@@ -482,6 +505,8 @@ bf_status bf_marching_cubes_table(uint8_t *triangle_count, int8_t *edges) {
 }
 
 const char *bf_marching_cubes_last_error(void) { return g_mc_error.c_str(); }
+
+double bf_marching_cubes_last_device_ms(void) { return g_mc_device_ms; }
 
 bf_status bf_marching_cubes(int32_t device, const float *volume, const int32_t resolution[3], float threshold,
                             const float mesh_scale[3], float *vertices, int64_t vertex_capacity, uint32_t *indices,

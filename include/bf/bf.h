@@ -331,6 +331,10 @@ bf_status bf_marching_cubes_table(uint8_t *triangle_count, int8_t *edges);
 
 const char *bf_marching_cubes_last_error(void);
 
+/* Device time (ms, HIP events) of the last extraction's passes on this thread:
+ * classification, prefix sums and the two emit passes (without transfers). */
+double bf_marching_cubes_last_device_ms(void);
+
 /* --- diagnostics (no counterpart in the reference) ---------------------- */
 
 /* Device copy bandwidth: copies `bytes` between two device buffers `iters`
