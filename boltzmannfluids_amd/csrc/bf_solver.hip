@@ -54,6 +54,7 @@ struct bf_solver {
     int object_count = 1;  // objects_cpu.size() in the reference (slot 0 = fluid)
     float tau = 0.53f, G = -6.0f, beta = 0.5f, tau_t = 2.0f, t_ref = 1.0f, cs2_t = 1.0f / 3.0f;
     float ulim = 0.25f, uext = 0.30f;
+    int sc_sync = 0;  // synchronous Shan-Chen pseudopotential (density pre-pass per tick)
     float fconst[3] = {0, 0, 0};
     int forcing = 0, thermal = 0, has_ids = 0, initialized = 0;
     int cur = 0;
@@ -545,6 +546,7 @@ bf_status bf_set_param(bf_solver *s, int32_t key, const float *v, int32_t n) {
         s->cs2_t = v[0]; return BF_OK;
     case BF_PARAM_VELOCITY_LIMIT: s->ulim = v[0]; return BF_OK;
     case BF_PARAM_VELOCITY_LIMIT_EXTREME: s->uext = v[0]; return BF_OK;
+    case BF_PARAM_SHAN_CHEN_SYNCHRONOUS: s->sc_sync = v[0] != 0.0f; return BF_OK;
     }
     return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: unknown key %d", key);
 }
@@ -563,6 +565,7 @@ bf_status bf_get_param(const bf_solver *s, int32_t key, float *v, int32_t n) {
     case BF_PARAM_THERMAL_CS2: v[0] = s->cs2_t; return BF_OK;
     case BF_PARAM_VELOCITY_LIMIT: v[0] = s->ulim; return BF_OK;
     case BF_PARAM_VELOCITY_LIMIT_EXTREME: v[0] = s->uext; return BF_OK;
+    case BF_PARAM_SHAN_CHEN_SYNCHRONOUS: v[0] = (float)s->sc_sync; return BF_OK;
     }
     return BF_ERR_INVALID;
 }
@@ -573,7 +576,19 @@ static bf_status run_ticks(bf_solver *s, int64_t n, uint32_t flags) {
         StepArgs a = make_args(s, true);
         if (last && (flags & BF_STEP_SAVE_MACRO)) { a.vd = s->vd; a.ft = s->ft; }
         double *partial = (last && (flags & BF_STEP_CONSERVATION)) ? s->partial : nullptr;
-        hipError_t e = dispatch_step(s, a, (s->ticks & 1) != 0, partial);
+        const bool odd = (s->ticks & 1) != 0;
+        if (s->cfg.multiphase && s->sc_sync) {
+            // density pre-pass: the same reads without writes of populations;
+            // psi of this tick's density goes to psi[cur], which the tick then
+            // reads as its neighbours' psi (and it writes its own to the other)
+            StepArgs p = make_args(s, false);
+            p.psi_cur = s->psi[s->cur];
+            const hipError_t e = dispatch_step(s, p, odd, nullptr);
+            if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] density pre-pass launch failed: %s", hipGetErrorString(e));
+            a.psi_prev = s->psi[s->cur];
+            a.psi_cur = s->psi[s->cur ^ 1];
+        }
+        hipError_t e = dispatch_step(s, a, odd, partial);
         if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] tick kernel launch failed: %s", hipGetErrorString(e));
         if (partial) {
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);

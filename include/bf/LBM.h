@@ -270,6 +270,7 @@ public:
     bool is_lattice_texture3d = false;        // storage layout is chosen by the library
     bool is_collide_esoteric = true;          // the update is always in place (AA pattern)
     bool compat_fullway_bounce_back = false;  // reproduce the reference's implicit full-way walls
+    bool shan_chen_synchronous = false;       // extension: current-tick neighbour psi (default: the reference's lag)
     float velocity_limit = 0.25f;
     float velocity_limit_extreme = 0.30f;
     float thermal_relaxation_time = 2.0f;
@@ -299,10 +300,12 @@ private:
     void push_params() {
         const float values[][1] = {{relaxation_time}, {intermolecular_interaction_strength},
                                    {thermal_expension_coeficient}, {thermal_relaxation_time},
-                                   {referance_temperature_value}, {velocity_limit}, {velocity_limit_extreme}};
+                                   {referance_temperature_value}, {velocity_limit}, {velocity_limit_extreme},
+                                   {shan_chen_synchronous ? 1.0f : 0.0f}};
         const int keys[] = {BF_PARAM_RELAXATION_TIME, BF_PARAM_INTERACTION_STRENGTH, BF_PARAM_THERMAL_EXPANSION,
                             BF_PARAM_THERMAL_RELAXATION_TIME, BF_PARAM_REFERENCE_TEMPERATURE,
-                            BF_PARAM_VELOCITY_LIMIT, BF_PARAM_VELOCITY_LIMIT_EXTREME};
+                            BF_PARAM_VELOCITY_LIMIT, BF_PARAM_VELOCITY_LIMIT_EXTREME,
+                            BF_PARAM_SHAN_CHEN_SYNCHRONOUS};
         for (size_t k = 0; k < sizeof(keys) / sizeof(keys[0]); ++k) check(bf_set_param(solver_, keys[k], values[k], 1), "set_param");
     }
     std::vector<float> read_floats(int field) {

@@ -97,7 +97,14 @@ typedef enum bf_param {
     BF_PARAM_REFERENCE_TEMPERATURE = 5,  /* 1 float T_ref (default 1); collide.comp:713 */
     BF_PARAM_THERMAL_CS2 = 6,            /* 1 float c_s^2 of the thermal set (default 1/3); LBM.cpp:1611 */
     BF_PARAM_VELOCITY_LIMIT = 7,         /* 1 float (default 0.25); LBM.h:144 */
-    BF_PARAM_VELOCITY_LIMIT_EXTREME = 8  /* 1 float (default 0.30); LBM.h:145 */
+    BF_PARAM_VELOCITY_LIMIT_EXTREME = 8, /* 1 float (default 0.30); LBM.h:145 */
+    /* 1 float, 0 or 1 (default 0).  0 = the reference's lagged neighbour
+     * pseudopotential (psi of the previous tick, collide.comp:537,567 with the
+     * ping-pong of LBM.cpp:1617-1621,1649-1650); 1 = synchronous: a density
+     * pre-pass per tick gives every neighbour's psi of the current tick
+     * (costs one extra read of the lattice per tick).  No counterpart in the
+     * reference. */
+    BF_PARAM_SHAN_CHEN_SYNCHRONOUS = 9
 } bf_param;
 
 /* Fields for bf_read_field. */

@@ -184,6 +184,7 @@ typedef struct orc_solver {
     int q, qt;
     const dir_t *T, *TT;
     int vset, storage, multiphase, bb_mode;
+    int sc_sync;       /* synchronous Shan-Chen pseudopotential (density pre-pass) */
     int per[3];
     /* parameters (reference defaults: LBM.h:144-145,243,257; LBM.cpp:461; collide.comp:713) */
     REAL tau, G, beta, tau_t, t_ref, cs2_t, ulim, uext;
@@ -349,10 +350,26 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
     int odd = (int)(s->t & 1);
     const REAL *psi_prev = s->multiphase ? s->psi[s->cur ^ 1] : NULL;
     REAL *psi_cur = s->multiphase ? s->psi[s->cur] : NULL;
+    const long ncell = (long)s->X * s->Y * s->Z;
+    if (s->multiphase && s->sc_sync && update) {
+        /* synchronous pseudopotential: psi of the density this tick streams in,
+         * for every fluid cell, before any cell collides */
+#pragma omp parallel for schedule(static)
+        for (long cc = 0; cc < ncell; ++cc) {
+            int x = (int)(cc % s->X), y = (int)((cc / s->X) % s->Y), z = (int)(cc / ((long)s->X * s->Y));
+            long cell = cid(s, x, y, z);
+            if (is_solid(s, cell)) continue;
+            REAL rho = 0;
+            for (int i = 0; i < s->q; ++i) rho += read_pop(s, x, y, z, i, odd);
+            if (rho < (REAL)0.01) rho = (REAL)0.01;
+            psi_cur[cell] = psi_of(rho);
+        }
+        psi_prev = s->psi[s->cur];
+        psi_cur = s->psi[s->cur ^ 1];
+    }
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
     /* each cell reads exactly the slots it writes (AA pattern), so cells are
      * independent and the loop runs in parallel when built with OpenMP */
-    const long ncell = (long)s->X * s->Y * s->Z;
 #pragma omp parallel for schedule(static) reduction(+ : s0, s1, s2, s3, s4)
     for (long cc = 0; cc < ncell; ++cc) {
         REAL f[27], g[7];
@@ -520,6 +537,7 @@ ORC_API int orc_set_param(orc_solver *s, int key, const double *v, int n) {
     case 6: s->cs2_t = (REAL)v[0]; return 0;
     case 7: s->ulim = (REAL)v[0]; return 0;
     case 8: s->uext = (REAL)v[0]; return 0;
+    case 9: s->sc_sync = v[0] != 0; return 0;
     }
     return 1;
 }

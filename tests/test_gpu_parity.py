@@ -187,6 +187,29 @@ def test_shan_chen_d2q9_with_wall(bfa, orc):
     assert_fp32_parity(g, o, atol=1e-5)
 
 
+@pytest.mark.parametrize("vs,shape,periodic", [("D3Q27", (16, 16, 16), (1, 1, 1)), ("D2Q9", (70, 20, 1), (1, 0, 1))])
+def test_shan_chen_synchronous(bfa, orc, vs, shape, periodic):
+    """Synchronous pseudopotential (density pre-pass per tick) against the
+    oracle's; it must also differ from the lagged run of the same case."""
+    N = shape[0] * shape[1] * shape[2]
+    rng = np.random.default_rng(SEED + 11)
+    rho = (0.3 + 0.05 * rng.random(N)).astype(np.float32)
+    rho.reshape(shape[2], shape[1], shape[0])[:, 5:12, 4:12] = 2.2
+    init = dict(rho=rho, u=np.zeros((N, 3), np.float32))
+    objects = []
+    if periodic[1] == 0:
+        init["boundary_id"] = channel_ids(shape[0], shape[1])
+        objects = [(1, dict(effective_density=1.5))]
+    params = [("interaction_strength", -5.0), ("shan_chen_synchronous", 1.0)]
+    g, o = run_pair(bfa, orc, shape, 0.9, 31, velocity_set=vs, multiphase=True, periodic=periodic,
+                    init=init, objects=objects, params=params)
+    assert_fp32_parity(g, o, atol=1e-5)
+    assert np.abs(g.read("psi") - o.read("psi")).max() < 1e-5
+    lag, _ = run_pair(bfa, orc, shape, 0.9, 31, velocity_set=vs, multiphase=True, periodic=periodic,
+                      init=init, objects=objects, params=params[:1])
+    assert np.abs(lag.read("velocity_density") - g.read("velocity_density")).max() > 1e-6
+
+
 def test_thermal_d3q19_rayleigh_benard_like(bfa, orc):
     X, Y, Z = 20, 16, 12
     N = X * Y * Z
