@@ -111,6 +111,10 @@ def load_library(path=None):
         "bf_unpack_boundaries": (i32, [C.POINTER(C.c_uint32), i64, i32, C.POINTER(C.c_uint8)]),
         "bf_probe_copy_bandwidth": (i32, [i32, i64, i32, C.POINTER(C.c_double)]),
         "bf_probe_inplace_bandwidth": (i32, [i32, i64, i32, i32, C.POINTER(C.c_double)]),
+        "bf_save_mesoscopic": (i32, [vp, C.c_char_p]),
+        "bf_save_macroscopic": (i32, [vp, C.c_char_p]),
+        "bf_load_mesoscopic": (i32, [vp, C.c_char_p, i64]),
+        "bf_load_macroscopic": (i32, [vp, C.c_char_p, i64]),
         "bf_marching_cubes_capacity": (i32, [C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "bf_marching_cubes": (i32, [i32, f32p, C.POINTER(C.c_int32), C.c_float, f32p, f32p, i64,
                                     C.POINTER(C.c_uint32), i64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -358,6 +362,19 @@ class Solver:
         i = Info()
         self._check(self._lib.bf_get_info(self._h, C.byref(i)), "bf_get_info")
         return {k: (list(getattr(i, k)) if k == "resolution" else getattr(i, k)) for k, _ in Info._fields_}
+
+    # checkpoint files: <directory>/tick_<ticks>.bfmeso / .bfmacro
+    def save_mesoscopic(self, directory):
+        self._check(self._lib.bf_save_mesoscopic(self._h, os.fsencode(directory)), "bf_save_mesoscopic")
+
+    def save_macroscopic(self, directory):
+        self._check(self._lib.bf_save_macroscopic(self._h, os.fsencode(directory)), "bf_save_macroscopic")
+
+    def load_mesoscopic(self, directory, tick):
+        self._check(self._lib.bf_load_mesoscopic(self._h, os.fsencode(directory), int(tick)), "bf_load_mesoscopic")
+
+    def load_macroscopic(self, directory, tick):
+        self._check(self._lib.bf_load_macroscopic(self._h, os.fsencode(directory), int(tick)), "bf_load_macroscopic")
 
     def isosurface(self, threshold, mesh_scale=(1.0, 1.0, 1.0)):
         """Marching-cubes iso-surface of the density saved on the last save

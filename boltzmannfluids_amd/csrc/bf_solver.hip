@@ -17,6 +17,8 @@
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "../../include/bf/bf.h"
 #include "bf_kernels.h"
 #include "bf_lattice.h"
@@ -59,6 +61,7 @@ struct bf_solver {
     int forcing = 0, thermal = 0, has_ids = 0, initialized = 0;
     int cur = 0;
     int64_t ticks = 0;
+    int64_t tick_base = 0;  // tick count at the last equilibrium initialisation: AA parity = ticks - tick_base
     LaunchShape shape{};
     std::string err;
 };
@@ -512,6 +515,7 @@ bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const f
     // save the macroscopic fields once without updating (LBM.cpp:502)
     s->cur = 0;
     s->ticks = 0;
+    s->tick_base = 0;
     StepArgs a = make_args(s, false);
     a.vd = s->vd; a.ft = s->ft;
     e = dispatch_step(s, a, false, nullptr);
@@ -576,7 +580,7 @@ static bf_status run_ticks(bf_solver *s, int64_t n, uint32_t flags) {
         StepArgs a = make_args(s, true);
         if (last && (flags & BF_STEP_SAVE_MACRO)) { a.vd = s->vd; a.ft = s->ft; }
         double *partial = (last && (flags & BF_STEP_CONSERVATION)) ? s->partial : nullptr;
-        const bool odd = (s->ticks & 1) != 0;
+        const bool odd = ((s->ticks - s->tick_base) & 1) != 0;
         if (s->cfg.multiphase && s->sc_sync) {
             // density pre-pass: the same reads without writes of populations;
             // psi of this tick's density goes to psi[cur], which the tick then
@@ -719,7 +723,7 @@ bf_status bf_conservation(bf_solver *s, double out[5]) {
     BF_HIP(s, hipSetDevice(s->cfg.device));
     StepArgs a = make_args(s, false);
     a.write_psi = 0;
-    hipError_t e = dispatch_step(s, a, (s->ticks & 1) != 0, s->partial);
+    hipError_t e = dispatch_step(s, a, ((s->ticks - s->tick_base) & 1) != 0, s->partial);
     if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] conservation kernel failed: %s", hipGetErrorString(e));
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);
     BF_HIP(s, hipGetLastError());
@@ -783,6 +787,284 @@ bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, u
         const int64_t b = c * bits;
         ids[c] = (uint8_t)((words[b / 32] >> (b % 32)) & mask);
     }
+    return BF_OK;
+}
+
+}  // extern "C"
+
+// ---- checkpoint files (LBM.h:123-126 declares them; the reference has no body) ----
+//
+// One file per save: <directory>/tick_<ticks>.bfmeso or .bfmacro.  Little
+// endian.  A fixed header (kind, grid, configuration, parameters, object table)
+// is followed by tagged sections (uint32 tag, uint64 byte count, bytes).
+// Mesoscopic files hold the raw state (populations as stored, in slot order and
+// the AA parity of the tick count; thermal populations; both pseudopotential buffers;
+// the force field; boundary ids; the saved macroscopic fields), so a loaded
+// solver continues bit for bit.  Macroscopic files hold the saved fields
+// (u, rho), (F, T) and the ids; loading one re-initialises the populations at
+// equilibrium like initialize_fields.
+
+namespace {
+
+constexpr char kMagic[8] = {'B', 'F', 'L', 'B', 'M', 'S', 'T', '1'};
+enum : uint32_t { kKindMacro = 0, kKindMeso = 1 };
+enum : uint32_t {
+    kTagPopulations = 1, kTagThermal = 2, kTagPsi = 3, kTagForce = 4, kTagIds = 5, kTagVelocityDensity = 6,
+    kTagForceTemperature = 7
+};
+
+struct FileHeader {
+    char magic[8];
+    uint32_t version, kind;
+    int32_t X, Y, Z, velocity_set, storage, multiphase, boundary_mode, periodic[3];
+    int32_t thermal, forcing, has_ids, object_count, cur, sc_sync;
+    int64_t ticks, tick_base;
+    float tau, G, beta, tau_t, t_ref, cs2_t, ulim, uext, fconst[3];
+    ObjectEntry objects[256];
+    float rho_eff[256];
+};
+
+std::string state_path(const char *dir, int64_t tick, uint32_t kind) {
+    return std::string(dir) + "/tick_" + std::to_string((long long)tick) + (kind == kKindMeso ? ".bfmeso" : ".bfmacro");
+}
+
+struct File {
+    FILE *f = nullptr;
+    ~File() {
+        if (f) std::fclose(f);
+    }
+};
+
+bool put(FILE *f, uint32_t tag, const void *p, uint64_t n) {
+    return std::fwrite(&tag, 4, 1, f) == 1 && std::fwrite(&n, 8, 1, f) == 1 && (n == 0 || std::fwrite(p, 1, n, f) == n);
+}
+
+// reads the next section; it must carry `tag` and exactly n bytes
+bool get(FILE *f, uint32_t tag, void *p, uint64_t n) {
+    uint32_t t = 0;
+    uint64_t m = 0;
+    return std::fread(&t, 4, 1, f) == 1 && std::fread(&m, 8, 1, f) == 1 && t == tag && m == n &&
+           (n == 0 || std::fread(p, 1, n, f) == n);
+}
+
+FileHeader header_of(const bf_solver *s, uint32_t kind) {
+    FileHeader h{};
+    std::memcpy(h.magic, kMagic, 8);
+    h.version = 1;
+    h.kind = kind;
+    h.X = s->X; h.Y = s->Y; h.Z = s->Z;
+    h.velocity_set = s->vs; h.storage = s->storage; h.multiphase = s->cfg.multiphase;
+    h.boundary_mode = s->cfg.boundary_mode;
+    for (int k = 0; k < 3; ++k) h.periodic[k] = s->cfg.periodic[k];
+    h.thermal = s->thermal; h.forcing = s->forcing; h.has_ids = s->has_ids; h.object_count = s->object_count;
+    h.cur = s->cur; h.sc_sync = s->sc_sync;
+    h.ticks = s->ticks;
+    h.tick_base = s->tick_base;
+    h.tau = s->tau; h.G = s->G; h.beta = s->beta; h.tau_t = s->tau_t; h.t_ref = s->t_ref; h.cs2_t = s->cs2_t;
+    h.ulim = s->ulim; h.uext = s->uext;
+    for (int k = 0; k < 3; ++k) h.fconst[k] = s->fconst[k];
+    std::memcpy(h.objects, s->objects_h, sizeof h.objects);
+    std::memcpy(h.rho_eff, s->rho_eff_h, sizeof h.rho_eff);
+    return h;
+}
+
+void apply_params(bf_solver *s, const FileHeader &h) {
+    s->tau = h.tau; s->G = h.G; s->beta = h.beta; s->tau_t = h.tau_t; s->t_ref = h.t_ref; s->cs2_t = h.cs2_t;
+    s->ulim = h.ulim; s->uext = h.uext; s->sc_sync = h.sc_sync;
+    for (int k = 0; k < 3; ++k) s->fconst[k] = h.fconst[k];
+    std::memcpy(s->objects_h, h.objects, sizeof h.objects);
+    std::memcpy(s->rho_eff_h, h.rho_eff, sizeof h.rho_eff);
+    s->object_count = h.object_count;
+}
+
+template <class T>
+bf_status download(bf_solver *s, std::vector<T> &out, const void *src, size_t bytes) {
+    out.resize((bytes + sizeof(T) - 1) / sizeof(T));
+    BF_HIP(s, hipMemcpy(out.data(), src, bytes, hipMemcpyDeviceToHost));
+    return BF_OK;
+}
+
+bf_status write_state(bf_solver *s, const char *dir, uint32_t kind) {
+    if (!s || !dir) return BF_ERR_INVALID;
+    if (!s->initialized) return fail(s, BF_ERR_STATE, "[LBM Error] save called before initialize_fields");
+    BF_HIP(s, hipSetDevice(s->cfg.device));
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    (void)mkdir(dir, 0755);
+    const std::string path = state_path(dir, s->ticks, kind);
+    File file;
+    file.f = std::fopen(path.c_str(), "wb");
+    if (!file.f) return fail(s, BF_ERR_INVALID, "[LBM Error] cannot write %s", path.c_str());
+    const FileHeader h = header_of(s, kind);
+    bool ok = std::fwrite(&h, sizeof h, 1, file.f) == 1;
+    const size_t N = (size_t)s->N;
+    std::vector<uint8_t> buf;
+    bf_status st;
+    auto section = [&](uint32_t tag, const void *dev, size_t bytes) -> bf_status {
+        if ((st = download(s, buf, dev, bytes)) != BF_OK) return st;
+        ok = ok && put(file.f, tag, buf.data(), bytes);
+        return BF_OK;
+    };
+    if (kind == kKindMeso) {
+        // populations slot by slot (the padding between slots is dropped)
+        std::vector<uint8_t> pops(s->q * N * s->elem);
+        for (int i = 0; i < s->q; ++i)
+            BF_HIP(s, hipMemcpy(pops.data() + i * N * s->elem, static_cast<char *>(s->f) + i * (size_t)s->stride * s->elem,
+                                N * s->elem, hipMemcpyDeviceToHost));
+        ok = ok && put(file.f, kTagPopulations, pops.data(), pops.size());
+        if (s->thermal) {
+            std::vector<uint8_t> th(s->qt * N * sizeof(float));
+            for (int k = 0; k < s->qt; ++k)
+                BF_HIP(s, hipMemcpy(th.data() + k * N * sizeof(float), s->g + (size_t)k * s->stride, N * sizeof(float),
+                                    hipMemcpyDeviceToHost));
+            ok = ok && put(file.f, kTagThermal, th.data(), th.size());
+        }
+        if (s->cfg.multiphase) {
+            std::vector<uint8_t> ps(2 * N * sizeof(float));
+            BF_HIP(s, hipMemcpy(ps.data(), s->psi[0], N * sizeof(float), hipMemcpyDeviceToHost));
+            BF_HIP(s, hipMemcpy(ps.data() + N * sizeof(float), s->psi[1], N * sizeof(float), hipMemcpyDeviceToHost));
+            ok = ok && put(file.f, kTagPsi, ps.data(), ps.size());
+        }
+        if (s->forcing == 2 && (st = section(kTagForce, s->force_d, N * sizeof(float4))) != BF_OK) return st;
+    }
+    if (s->has_ids && (st = section(kTagIds, s->ids, N)) != BF_OK) return st;
+    if ((st = section(kTagVelocityDensity, s->vd, N * sizeof(float4))) != BF_OK) return st;
+    if ((st = section(kTagForceTemperature, s->ft, N * sizeof(float4))) != BF_OK) return st;
+    if (!ok || std::fflush(file.f) != 0) return fail(s, BF_ERR_INVALID, "[LBM Error] writing %s failed", path.c_str());
+    return BF_OK;
+}
+
+bf_status open_state(bf_solver *s, const char *dir, int64_t tick, uint32_t kind, File &file, FileHeader &h) {
+    const std::string path = state_path(dir, tick, kind);
+    file.f = std::fopen(path.c_str(), "rb");
+    if (!file.f) return fail(s, BF_ERR_INVALID, "[LBM Error] no checkpoint %s", path.c_str());
+    if (std::fread(&h, sizeof h, 1, file.f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1 ||
+        h.kind != kind)
+        return fail(s, BF_ERR_INVALID, "[LBM Error] %s is not a checkpoint of this kind", path.c_str());
+    if (h.X != s->X || h.Y != s->Y || h.Z != s->Z || h.velocity_set != s->vs || h.multiphase != s->cfg.multiphase ||
+        h.boundary_mode != s->cfg.boundary_mode || h.periodic[0] != s->cfg.periodic[0] ||
+        h.periodic[1] != s->cfg.periodic[1] || h.periodic[2] != s->cfg.periodic[2] ||
+        (kind == kKindMeso && h.storage != s->storage))
+        return fail(s, BF_ERR_INVALID, "[LBM Error] %s was saved by a solver of another configuration", path.c_str());
+    return BF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bf_status bf_save_mesoscopic(bf_solver *s, const char *directory) { return write_state(s, directory, kKindMeso); }
+bf_status bf_save_macroscopic(bf_solver *s, const char *directory) { return write_state(s, directory, kKindMacro); }
+
+bf_status bf_load_macroscopic(bf_solver *s, const char *directory, int64_t tick) {
+    if (!s || !directory) return BF_ERR_INVALID;
+    File file;
+    FileHeader h;
+    bf_status st = open_state(s, directory, tick, kKindMacro, file, h);
+    if (st != BF_OK) return st;
+    const size_t N = (size_t)s->N;
+    std::vector<uint8_t> ids;
+    if (h.has_ids) {
+        ids.resize(N);
+        if (!get(file.f, kTagIds, ids.data(), N)) return fail(s, BF_ERR_INVALID, "[LBM Error] truncated checkpoint");
+    }
+    std::vector<float> vd(4 * N), ft(4 * N);
+    if (!get(file.f, kTagVelocityDensity, vd.data(), 16 * N) || !get(file.f, kTagForceTemperature, ft.data(), 16 * N))
+        return fail(s, BF_ERR_INVALID, "[LBM Error] truncated checkpoint");
+    std::vector<float> rho(N), u(3 * N), force(3 * N), T(N);
+    for (size_t c = 0; c < N; ++c) {
+        rho[c] = vd[4 * c + 3];
+        for (int k = 0; k < 3; ++k) { u[3 * c + k] = vd[4 * c + k]; force[3 * c + k] = ft[4 * c + k]; }
+        T[c] = ft[4 * c + 3];
+    }
+    apply_params(s, h);
+    st = bf_init_fields(s, rho.data(), u.data(), h.forcing ? force.data() : nullptr, h.thermal ? T.data() : nullptr,
+                        h.has_ids ? ids.data() : nullptr);
+    if (st != BF_OK) return st;
+    // the populations restart at equilibrium in the layout of an even tick
+    s->ticks = h.ticks;
+    s->tick_base = h.ticks;
+    return BF_OK;
+}
+
+bf_status bf_load_mesoscopic(bf_solver *s, const char *directory, int64_t tick) {
+    if (!s || !directory) return BF_ERR_INVALID;
+    File file;
+    FileHeader h;
+    bf_status st = open_state(s, directory, tick, kKindMeso, file, h);
+    if (st != BF_OK) return st;
+    BF_HIP(s, hipSetDevice(s->cfg.device));
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    const size_t N = (size_t)s->N;
+    auto bad = [&]() { return fail(s, BF_ERR_INVALID, "[LBM Error] truncated or inconsistent checkpoint"); };
+    std::vector<uint8_t> pops(s->q * N * s->elem), th, ps, force, ids, vd(16 * N), ft(16 * N);
+    if (!get(file.f, kTagPopulations, pops.data(), pops.size())) return bad();
+    if (h.thermal) {
+        th.resize(s->qt * N * sizeof(float));
+        if (!get(file.f, kTagThermal, th.data(), th.size())) return bad();
+    }
+    if (h.multiphase) {
+        ps.resize(2 * N * sizeof(float));
+        if (!get(file.f, kTagPsi, ps.data(), ps.size())) return bad();
+    }
+    if (h.forcing == 2) {
+        force.resize(N * sizeof(float4));
+        if (!get(file.f, kTagForce, force.data(), force.size())) return bad();
+    }
+    if (h.has_ids) {
+        ids.resize(N);
+        if (!get(file.f, kTagIds, ids.data(), N)) return bad();
+    }
+    if (!get(file.f, kTagVelocityDensity, vd.data(), vd.size()) || !get(file.f, kTagForceTemperature, ft.data(), ft.size()))
+        return bad();
+    // device buffers for what the state uses
+    dfree(s->ids); s->ids = nullptr;
+    dfree(s->nmask); s->nmask = nullptr;
+    if (h.has_ids) {
+        if ((st = dalloc(s, &s->ids, N)) != BF_OK) return st;
+        if ((st = dalloc(s, &s->nmask, sizeof(uint32_t) * N)) != BF_OK) return st;
+        BF_HIP(s, hipMemcpy(s->ids, ids.data(), N, hipMemcpyHostToDevice));
+    }
+    dfree(s->force_d); s->force_d = nullptr;
+    if (h.forcing == 2) {
+        if ((st = dalloc(s, &s->force_d, force.size())) != BF_OK) return st;
+        BF_HIP(s, hipMemcpy(s->force_d, force.data(), force.size(), hipMemcpyHostToDevice));
+    }
+    dfree(s->g); s->g = nullptr;
+    if (h.thermal) {
+        if ((st = dalloc(s, &s->g, sizeof(float) * (size_t)s->qt * s->stride)) != BF_OK) return st;
+        for (int k = 0; k < s->qt; ++k)
+            BF_HIP(s, hipMemcpy(s->g + (size_t)k * s->stride, th.data() + k * N * sizeof(float), N * sizeof(float),
+                                hipMemcpyHostToDevice));
+    }
+    for (int i = 0; i < s->q; ++i)
+        BF_HIP(s, hipMemcpy(static_cast<char *>(s->f) + i * (size_t)s->stride * s->elem, pops.data() + i * N * s->elem,
+                            N * s->elem, hipMemcpyHostToDevice));
+    if (h.multiphase) {
+        BF_HIP(s, hipMemcpy(s->psi[0], ps.data(), N * sizeof(float), hipMemcpyHostToDevice));
+        BF_HIP(s, hipMemcpy(s->psi[1], ps.data() + N * sizeof(float), N * sizeof(float), hipMemcpyHostToDevice));
+    }
+    BF_HIP(s, hipMemcpy(s->vd, vd.data(), vd.size(), hipMemcpyHostToDevice));
+    BF_HIP(s, hipMemcpy(s->ft, ft.data(), ft.size(), hipMemcpyHostToDevice));
+    apply_params(s, h);
+    s->has_ids = h.has_ids;
+    s->thermal = h.thermal;
+    s->forcing = h.forcing;
+    s->cur = h.cur;
+    s->ticks = h.ticks;
+    s->tick_base = h.tick_base;
+    if ((st = upload_objects(s)) != BF_OK) return st;
+    if (s->has_ids) BF_HIP(s, dispatch_mask(s));
+    s->shape = step_shape(s->X, s->Y, s->Z, s->cfg.multiphase != 0,
+                          cells_per_thread(s->storage, s->cfg.multiphase != 0, s->thermal != 0), s->vs);
+    s->partial_blocks = (size_t)s->shape.grid.x * s->shape.grid.y * s->shape.grid.z;
+    if (s->partial_blocks > s->partial_capacity) {
+        dfree(s->partial);
+        s->partial = nullptr;
+        s->partial_capacity = s->partial_blocks;
+        if ((st = dalloc(s, &s->partial, sizeof(double) * 5 * s->partial_capacity)) != BF_OK) return st;
+    }
+    BF_HIP(s, hipStreamSynchronize(s->stream));
+    s->initialized = 1;
     return BF_OK;
 }
 

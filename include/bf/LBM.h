@@ -264,6 +264,29 @@ public:
     void set_intermolecular_interaction_strength(float value) { intermolecular_interaction_strength = value; }
 
     // The C handle, for callers that use the C ABI directly (interop, tests).
+    // checkpoint files (LBM.h:123-126 of the reference, which declares them
+    // without a body): <save_path>/tick_<n>.bfmacro / .bfmeso
+    void save_current_tick_macroscopic(const std::string &save_path) {
+        require_solver("save_current_tick_macroscopic");
+        check(bf_save_macroscopic(solver_, save_path.c_str()), "save_current_tick_macroscopic");
+    }
+    void save_current_tick_mesoscropic(const std::string &save_path) {
+        require_solver("save_current_tick_mesoscropic");
+        check(bf_save_mesoscopic(solver_, save_path.c_str()), "save_current_tick_mesoscropic");
+    }
+    void load_tick_macroscopic(const std::string &save_path, int32_t target_tick) {
+        require_solver("load_tick_macroscopic");
+        check(bf_load_macroscopic(solver_, save_path.c_str(), target_tick), "load_tick_macroscopic");
+        total_ticks_elapsed = (size_t)target_tick;
+        pull_params();
+    }
+    void load_tick_mesoscropic(const std::string &save_path, int32_t target_tick) {
+        require_solver("load_tick_mesoscropic");
+        check(bf_load_mesoscopic(solver_, save_path.c_str(), target_tick), "load_tick_mesoscropic");
+        total_ticks_elapsed = (size_t)target_tick;
+        pull_params();
+    }
+
     bf_solver *native_handle() { return solver_; }
 
     // --- public knobs of the reference (LBM.h:139-145, 242-257) ----------
@@ -296,6 +319,19 @@ private:
     }
     void require_solver(const char *what) {
         if (!solver_) fatal(std::string("[LBM Error] ") + what + " is called before initialize_fields()");
+    }
+    // the parameters a loaded checkpoint carries become the object's
+    void pull_params() {
+        float v = 0.0f;
+        check(bf_get_param(solver_, BF_PARAM_RELAXATION_TIME, &v, 1), "get_param"); relaxation_time = v;
+        check(bf_get_param(solver_, BF_PARAM_INTERACTION_STRENGTH, &v, 1), "get_param");
+        intermolecular_interaction_strength = v;
+        check(bf_get_param(solver_, BF_PARAM_THERMAL_EXPANSION, &v, 1), "get_param"); thermal_expension_coeficient = v;
+        check(bf_get_param(solver_, BF_PARAM_THERMAL_RELAXATION_TIME, &v, 1), "get_param"); thermal_relaxation_time = v;
+        check(bf_get_param(solver_, BF_PARAM_REFERENCE_TEMPERATURE, &v, 1), "get_param"); referance_temperature_value = v;
+        check(bf_get_param(solver_, BF_PARAM_VELOCITY_LIMIT, &v, 1), "get_param"); velocity_limit = v;
+        check(bf_get_param(solver_, BF_PARAM_VELOCITY_LIMIT_EXTREME, &v, 1), "get_param"); velocity_limit_extreme = v;
+        check(bf_get_param(solver_, BF_PARAM_SHAN_CHEN_SYNCHRONOUS, &v, 1), "get_param"); shan_chen_synchronous = v != 0.0f;
     }
     void push_params() {
         const float values[][1] = {{relaxation_time}, {intermolecular_interaction_strength},

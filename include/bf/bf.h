@@ -300,6 +300,24 @@ bf_status bf_channel_read_moments(bf_channel *c, float *out, size_t bytes);
  * [1] recomputation, [2] whole gradient (0 for a forward-only run). */
 bf_status bf_channel_timing(const bf_channel *c, double out[3]);
 
+/* --- checkpoint files ----------------------------------------------------
+ * LBM::save_current_tick_macroscopic / save_current_tick_mesoscropic /
+ * load_tick_macroscopic / load_tick_mesoscropic (LBM.h:123-126), declared but
+ * never implemented by the reference.  One file per save in `directory`
+ * (created if missing): tick_<ticks>.bfmeso or tick_<ticks>.bfmacro.
+ * Mesoscopic files hold the raw state (populations as stored, thermal
+ * populations, pseudopotential buffers, force field, ids, parameters and the
+ * object table): loading one into a solver of the same configuration continues
+ * the run bit for bit.  Macroscopic files hold the saved fields (u, rho),
+ * (F, T) and the ids: loading one re-initialises the populations at
+ * equilibrium (bf_init_fields) and sets the tick count.  Loading checks that
+ * the file's grid, velocity set, multiphase flag, boundary mode, periodicity
+ * (and storage, for mesoscopic files) match the solver. */
+bf_status bf_save_mesoscopic(bf_solver *solver, const char *directory);
+bf_status bf_save_macroscopic(bf_solver *solver, const char *directory);
+bf_status bf_load_mesoscopic(bf_solver *solver, const char *directory, int64_t tick);
+bf_status bf_load_macroscopic(bf_solver *solver, const char *directory, int64_t tick);
+
 /* --- iso-surface (marching cubes) ---------------------------------------
  * Replaces MarchingCubes::compute (MarchingCubes/MarchingCubes.cpp:3-48,
  * GLSL/MarchingCubes/marching_cubes.comp:41-98), which the reference never
