@@ -48,7 +48,7 @@ def build(verbose=False, jobs=None, out=None, defines=()):
     for name, vs in VELOCITY_SETS.items():
         obj = os.path.join(build_dir, f"bf_step_{name}.o")
         units.append([HIPCC, *COMMON, *extra, f"-DBF_VS={vs}", "-c", os.path.join(CSRC, "bf_step.hip"), "-o", obj])
-    for src in ("bf_solver", "bf_probe", "bf_channel", "bf_mc"):
+    for src in ("bf_solver", "bf_probe", "bf_channel", "bf_mc", "bf_voxel"):
         units.append([HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src + ".hip"),
                       "-o", os.path.join(build_dir, src + ".o")])
     with ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -115,6 +115,9 @@ def load_library(path=None):
         "bf_save_macroscopic": (i32, [vp, C.c_char_p]),
         "bf_load_mesoscopic": (i32, [vp, C.c_char_p, i64]),
         "bf_load_macroscopic": (i32, [vp, C.c_char_p, i64]),
+        "bf_voxelize_mesh": (i32, [i32, f32p, i64, C.POINTER(C.c_uint32), i64, f32p, C.POINTER(C.c_int32),
+                                   C.c_uint8, C.POINTER(C.c_uint8), C.POINTER(C.c_int64)]),
+        "bf_voxelize_last_error": (C.c_char_p, []),
         "bf_marching_cubes_capacity": (i32, [C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "bf_marching_cubes": (i32, [i32, f32p, C.POINTER(C.c_int32), C.c_float, f32p, f32p, i64,
                                     C.POINTER(C.c_uint32), i64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -258,6 +261,30 @@ def marching_cubes(volume, threshold, mesh_scale=(1.0, 1.0, 1.0), device=0):
     sc = np.ascontiguousarray(mesh_scale, np.float32)
     return _mc_run(lambda vp_, vc, ip, ic, nv, ni: lib.bf_marching_cubes(
         int(device), _fptr(v), res, float(threshold), _fptr(sc), vp_, vc, ip, ic, nv, ni))
+
+
+def voxelize_mesh(vertices, triangles, resolution, value=1, model=None, volume=None, device=0):
+    """Mark the voxels inside a closed mesh (vertices (V, 3), triangles (T, 3))
+    with `value` in a uint8 volume of shape (Z, Y, X) (a new zero volume when
+    None); `model` is a 4x4 matrix applied to the vertices (row-major numpy
+    layout, i.e. model @ [x, y, z, 1]).  Returns the volume."""
+    lib = load_library()
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    X, Y, Z = (int(r) for r in resolution)
+    if volume is None:
+        volume = np.zeros((Z, Y, X), np.uint8)
+    if volume.shape != (Z, Y, X) or volume.dtype != np.uint8 or not volume.flags.c_contiguous:
+        raise ValueError("volume must be a contiguous uint8 array of shape (Z, Y, X)")
+    m = np.eye(4, dtype=np.float32) if model is None else np.asarray(model, np.float32)
+    mc_ = np.ascontiguousarray(m.T)      # column-major for the C ABI
+    res = (C.c_int32 * 3)(X, Y, Z)
+    n = C.c_int64()
+    st = lib.bf_voxelize_mesh(int(device), _fptr(v), len(v), t.ctypes.data_as(C.POINTER(C.c_uint32)), len(t),
+                              _fptr(mc_), res, int(value), volume.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(n))
+    if st:
+        raise BFError(st, lib.bf_voxelize_last_error().decode())
+    return volume
 
 
 class Solver:

@@ -360,6 +360,26 @@ const char *bf_marching_cubes_last_error(void);
  * classification, prefix sums and the two emit passes (without transfers). */
 double bf_marching_cubes_last_device_ms(void);
 
+/* --- mesh voxeliser -------------------------------------------------------
+ * Replaces MeshRasterizer3D::rasterize (MeshRasterizer3D/MeshRasterizer3D.cpp:
+ * 7-100): marks the voxels inside a closed triangle mesh with `value` in
+ * `volume` (X*Y*Z bytes, cell order z*X*Y + y*X + x, updated in place, other
+ * voxels untouched), e.g. boundary ids for bf_init_fields.  `model` is a
+ * column-major 4x4 matrix (glm::mat4 layout) applied to the vertices; the grid
+ * spans the cube [-1, 1]^3 of model space as the reference's orthographic
+ * camera does: voxel (i, j, k) covers x in [-1 + 2i/X, -1 + 2(i+1)/X], y
+ * likewise and z in [1 - 2(k+1)/Z, 1 - 2k/Z] (slice 0 nearest +z).  A voxel is
+ * inside when its z range overlaps, along the line through its x-y centre, an
+ * interval between the 2m-th and (2m+1)-th crossing of the mesh.
+ * *solid_count (optional) receives the number of voxels holding `value`
+ * afterwards.  BF_ERR_UNSUPPORTED when a column crosses the mesh more than
+ * 64 times. */
+bf_status bf_voxelize_mesh(int32_t device, const float *vertices, int64_t vertex_count, const uint32_t *indices,
+                           int64_t triangle_count, const float model[16], const int32_t resolution[3], uint8_t value,
+                           uint8_t *volume, int64_t *solid_count);
+
+const char *bf_voxelize_last_error(void);
+
 /* --- diagnostics (no counterpart in the reference) ---------------------- */
 
 /* Device copy bandwidth: copies `bytes` between two device buffers `iters`
