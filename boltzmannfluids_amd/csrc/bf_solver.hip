@@ -683,11 +683,15 @@ bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes) {
         else std::memset(dst, 0, N);
         return BF_OK;
     case BF_FIELD_POPULATIONS_RAW: {
+        // the lattice holds h = f - w: add the weight back on the host
         float *out = static_cast<float *>(dst);
         if (s->storage == ST_FP32) {
-            for (int i = 0; i < s->q; ++i)
+            for (int i = 0; i < s->q; ++i) {
                 BF_HIP(s, hipMemcpy(out + i * N, static_cast<float *>(s->f) + (size_t)i * s->stride, sizeof(float) * N,
                                     hipMemcpyDeviceToHost));
+                const float w = weight_of(s->vs, i);
+                for (size_t c = 0; c < N; ++c) out[i * N + c] += w;
+            }
             return BF_OK;
         }
         std::vector<uint16_t> tmp(N);
@@ -807,6 +811,8 @@ bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, u
 namespace {
 
 constexpr char kMagic[8] = {'B', 'F', 'L', 'B', 'M', 'S', 'T', '1'};
+// version 2: the population section holds h = f - w for every storage type
+constexpr uint32_t kVersion = 2;
 enum : uint32_t { kKindMacro = 0, kKindMeso = 1 };
 enum : uint32_t {
     kTagPopulations = 1, kTagThermal = 2, kTagPsi = 3, kTagForce = 4, kTagIds = 5, kTagVelocityDensity = 6,
@@ -850,7 +856,7 @@ bool get(FILE *f, uint32_t tag, void *p, uint64_t n) {
 FileHeader header_of(const bf_solver *s, uint32_t kind) {
     FileHeader h{};
     std::memcpy(h.magic, kMagic, 8);
-    h.version = 1;
+    h.version = kVersion;
     h.kind = kind;
     h.X = s->X; h.Y = s->Y; h.Z = s->Z;
     h.velocity_set = s->vs; h.storage = s->storage; h.multiphase = s->cfg.multiphase;
@@ -937,7 +943,7 @@ bf_status open_state(bf_solver *s, const char *dir, int64_t tick, uint32_t kind,
     const std::string path = state_path(dir, tick, kind);
     file.f = std::fopen(path.c_str(), "rb");
     if (!file.f) return fail(s, BF_ERR_INVALID, "[LBM Error] no checkpoint %s", path.c_str());
-    if (std::fread(&h, sizeof h, 1, file.f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1 ||
+    if (std::fread(&h, sizeof h, 1, file.f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != kVersion ||
         h.kind != kind)
         return fail(s, BF_ERR_INVALID, "[LBM Error] %s is not a checkpoint of this kind", path.c_str());
     if (h.X != s->X || h.Y != s->Y || h.Z != s->Z || h.velocity_set != s->vs || h.multiphase != s->cfg.multiphase ||

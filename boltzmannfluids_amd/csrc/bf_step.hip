@@ -77,30 +77,31 @@ __device__ inline T *at(T *p, uint32_t idx) {
     return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + idx * (uint32_t)sizeof(T));
 }
 
-// Storage policies: FP32 as is; FP16 and BF16 hold f - w.  `raw` issues the
-// load, `cvt` turns the stored value into f; the tick issues all raw loads of a
-// cell before it converts any, so the loads stay in flight together.
+// Storage policies.  Every storage type holds h = f - w (the population minus
+// its rest-state weight), and the tick computes on h: at low Mach number f is
+// w plus a small deviation, so the deviation keeps all 24 bits of an FP32
+// mantissa (and all 11 or 8 bits of FP16 or BF16) instead of losing the bits
+// that w occupies.  `raw` issues the load, `cvt` widens it to FP32; the tick
+// issues all raw loads of a cell before it converts any, so the loads stay in
+// flight together.
 template <int ST> struct Store;
 template <> struct Store<ST_FP32> {
     using T = float;
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
-    __device__ static float cvt(T v, float) { return v; }
-    __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, uint32_t idx, float v, float) { st(at(p, idx), v); }
+    __device__ static float cvt(T v) { return v; }
+    __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), h); }
 };
 template <> struct Store<ST_FP16> {
     using T = _Float16;
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
-    __device__ static float cvt(T v, float w) { return (float)v + w; }
-    __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(at(p, idx), (_Float16)(v - w)); }
+    __device__ static float cvt(T v) { return (float)v; }
+    __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), (_Float16)h); }
 };
 template <> struct Store<ST_BF16> {
     using T = uint16_t;
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
-    __device__ static float cvt(T v, float w) { return __uint_as_float((uint32_t)v << 16) + w; }
-    __device__ static float load(const T *p, uint32_t idx, float w) { return cvt(raw(p, idx), w); }
-    __device__ static void store(T *p, uint32_t idx, float v, float w) { st(at(p, idx), bf16_rne(v - w)); }
+    __device__ static float cvt(T v) { return __uint_as_float((uint32_t)v << 16); }
+    __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), bf16_rne(h)); }
 };
 
 // c_i . v summed over the non-zero components of c_i in x, y, z order.  Equal
@@ -320,7 +321,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
-                LDS(i, xs[k][1]) = S::cvt(raw[k][i], L::w[i]);
+                LDS(i, xs[k][1]) = S::cvt(raw[k][i]);
             }
         }
     }
@@ -332,19 +333,19 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             const float w = L::w[i];
             const int o = opposite(i);
             if (ODD || i == 0) {
-                fi[k][i] = S::cvt(raw[k][i], L::w[ODD ? o : 0]);
+                fi[k][i] = S::cvt(raw[k][i]);
                 continue;
             }
             const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
             const bool valid = vx[k][dx] && vy[dy] && vz[dz];
             const bool redir = a.halfway && SOLID(k, o);
             if (!valid) {
-                fi[k][i] = 0.01f;  // open boundary read (collide.comp:297-298)
+                fi[k][i] = 0.01f - w;  // open boundary read f = 0.01 (collide.comp:297-298)
             } else if (redir) {
                 // moving wall (Ladd): + 6 w rho_w (c_i . u_w), rho_w = 1; a wall at
                 // rest adds exactly zero.  (Aligned x-directions: after the row exchange.)
                 if (!(AL && L::c[i][0] != 0)) {
-                    float v = S::cvt(raw[k][i], L::w[o]);
+                    float v = S::cvt(raw[k][i]);
                     if (a.wall_ids) {
                         const ObjectEntry ob = WALL(k, o);
                         v = __builtin_fmaf(6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
@@ -352,7 +353,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     fi[k][i] = v;
                 }
             } else if (!(AL && L::c[i][0] != 0)) {
-                fi[k][i] = S::cvt(raw[k][i], w);
+                fi[k][i] = S::cvt(raw[k][i]);
             }
         }
         if constexpr (TH) {
@@ -413,7 +414,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     if (L::c[i][0] == 0) continue;
                     const int o = opposite(i);
                     if (!(active[k] && a.halfway && SOLID(k, o))) continue;
-                    float v = S::cvt(rr[i], L::w[o]);
+                    float v = S::cvt(rr[i]);
                     if (a.wall_ids) {
                         const ObjectEntry ob = WALL(k, o);
                         v = __builtin_fmaf(6.0f * L::w[i], dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
@@ -428,10 +429,12 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         if (!active[k]) continue;
-        // moments (collide.comp:383-396, 704-705)
-        float rho = fi[k][0];
+        // moments (collide.comp:383-396, 704-705): d = rho - 1 = sum of h_i;
+        // sum c_i w_i = 0, so the momentum is the same sum over h as over f
+        float d = fi[k][0];
 #pragma unroll
-        for (int i = 1; i < Q; ++i) rho += fi[k][i];
+        for (int i = 1; i < Q; ++i) d += fi[k][i];
+        float rho = 1.0f + d;
         float mx = 0.0f, my = 0.0f, mz = 0.0f;
         {
             bool fx0 = true, fy0 = true, fz0 = true;
@@ -443,8 +446,12 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? v : -v; mz = fz0 ? t : mz + t; fz0 = false; }
             }
         }
-        if (partial) { s_rho += rho; s_mx += mx; s_my += my; s_mz += mz; }
-        rho = fmaxf(rho, 0.01f);
+        if (partial) { s_rho += 1.0 + (double)d; s_mx += mx; s_my += my; s_mz += mz; }
+        {
+            const bool low = !(rho >= 0.01f);   // density floor 0.01 (also taken by NaN)
+            rho = low ? 0.01f : rho;
+            d = low ? -0.99f : d;
+        }
 
         // force (collide.comp:707-720)
         float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
@@ -497,7 +504,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         if (a.update) {
             const float uu = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
             const float fu = __builtin_fmaf(Fz, uz, __builtin_fmaf(Fy, uy, Fx * ux));
-            const float c1 = __builtin_fmaf(-1.5f, uu, 1.0f);
+            const float c1 = -1.5f * uu;
             const float inv_tau = 1.0f / a.tau;
             const float pre = 1.0f - 1.0f / (2.0f * a.tau);
 #pragma unroll
@@ -505,20 +512,21 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 const float w = L::w[i];
                 const int o = opposite(i);
                 const float cu = (i == 0) ? 0.0f : dotc<BF_VS>(i, ux, uy, uz);
-                // w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2)
-                const float e = (w * rho) * __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1);
+                // h_eq = w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2) - w
+                //      = (w rho)(cu (3 + 9/2 cu) - 3/2 u^2) + w d
+                const float e = __builtin_fmaf(w * rho, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1), w * d);
                 float v;
                 if (ext) {
                     v = e;
                 } else {
-                    v = __builtin_fmaf(e - fi[k][i], inv_tau, fi[k][i]);   // f - (f - feq)/tau
+                    v = __builtin_fmaf(e - fi[k][i], inv_tau, fi[k][i]);   // h - (h - h_eq)/tau
                     if (a.source_on) {
                         const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
                         v += (pre * w) * (__builtin_fmaf(9.0f * cf, cu, 3.0f * cf) - 3.0f * fu);
                     }
                 }
                 if (ODD || i == 0) {
-                    S::store(SLOT(i), self[k], v, w);
+                    S::store(SLOT(i), self[k], v);
                 } else {
                     const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                     if (!OK(k, dx, dy, dz)) {
@@ -529,11 +537,11 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                             const ObjectEntry ob = WALL(k, i);
                             vb = __builtin_fmaf(-6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
                         }
-                        S::store(SLOT(i), self[k], vb, w);
+                        S::store(SLOT(i), self[k], vb);
                     } else if (AL && L::c[i][0] != 0) {
                         LDS(i, xs[k][dx]) = v;
                     } else {
-                        S::store(SLOT(o), xs[k][dx] + ry[dy] + pz[dz], v, w);
+                        S::store(SLOT(o), xs[k][dx] + ry[dy] + pz[dz], v);
                     }
                 }
             }
@@ -594,7 +602,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 const float v = LDS(i, x);
                 if (__float_as_uint(v) == kNoValue) continue;
                 const int dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
-                S::store(SLOT(opposite(i)), x + ry[dy] + pz[dz], v, L::w[i]);
+                S::store(SLOT(opposite(i)), x + ry[dy] + pz[dz], v);
             }
         }
     }
@@ -649,12 +657,16 @@ __global__ void __launch_bounds__(256) k_init(const InitArgs a) {
     const uint8_t id = a.ids ? a.ids[c] : 0;
     if (id) { ux = 0.0f; uy = 0.0f; uz = 0.0f; }
     const float uu = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
-    const float c1 = __builtin_fmaf(-1.5f, uu, 1.0f);
+    const float c1 = -1.5f * uu, d = r - 1.0f;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         const float cu = i == 0 ? 0.0f : dotc<VS>(i, ux, uy, uz);
-        const float e = (L::w[i] * r) * __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1);
-        S::store(f + (size_t)i * a.stride, (uint32_t)c, e, L::w[i]);
+        const float w = L::w[i];
+        float e = __builtin_fmaf(w * r, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1), w * d);
+        // round to FP32 before the storage conversion: without the empty asm the
+        // FMA and an FP16 conversion fuse into v_fma_mixlo_f16, which rounds once
+        if constexpr (ST == ST_FP16) asm("" : "+v"(e));
+        S::store(f + (size_t)i * a.stride, (uint32_t)c, e);
     }
     if constexpr (TH) {
         const float T0 = a.temperature ? a.temperature[c] : 1.0f;

@@ -16,8 +16,9 @@
  *     ticks it reads slot opp(i) at x and writes slot i at x;
  *   - BGK collision with Guo forcing, velocity clamp and density floor as in
  *     collide.comp:676-793;
- *   - FP16/BF16 storage keeps the shifted value f_i - w_i and does all
- *     arithmetic in FP32.
+ *   - every storage type keeps the shifted value h_i = f_i - w_i and the
+ *     tick computes on h in FP32 (rho = 1 + sum h_i), so low-Mach deviations
+ *     keep the full mantissa; FP16/BF16 storage converts to FP32 on load.
  */
 #ifndef BF_BF_H
 #define BF_BF_H
@@ -58,7 +59,7 @@ typedef enum bf_thermal_set {
 
 /* Population storage.  BF_FP32 and BF_FP16 keep the numeric values of enum
  * FloatingPointAccuracy (FloatingPointAccuracy.h:6-9); the reference stores
- * FP16 unshifted, this library stores f - w (shifted) for both 16-bit types. */
+ * f unshifted, this library stores h = f - w (shifted) for every type. */
 typedef enum bf_storage {
     BF_STORE_FP16 = 0,
     BF_STORE_FP32 = 1,
@@ -117,7 +118,7 @@ typedef enum bf_field {
     /* float4 per cell (F.x, F.y, F.z, T); force_temperature_texture (collide.comp:790). */
     BF_FIELD_FORCE_TEMPERATURE = 2,
     /* Q*N floats: the in-place lattice exactly as stored (AA slot layout,
-     * population-major), converted to unshifted FP32. */
+     * population-major), converted to unshifted FP32 (h + w in FP32). */
     BF_FIELD_POPULATIONS_RAW = 3,
     /* Qt*N floats: the in-place thermal lattice as stored (AA slot layout). */
     BF_FIELD_THERMAL_RAW = 4,

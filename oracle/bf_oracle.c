@@ -9,8 +9,11 @@
  *   REAL=double -> liboracle_f64.so: the absolute FP64 oracle (populations
  *                  stored in double whatever storage type is requested);
  *   REAL=float  -> liboracle_f32.so: FP32 arithmetic in the same operation
- *                  order as the GPU kernel, with FP16/BF16 shifted storage
- *                  emulated bit for bit (round to nearest even of f - w).
+ *                  order as the GPU kernel, with FP16/BF16 storage emulated
+ *                  bit for bit (round to nearest even).
+ * Both store and compute on the shifted population h_i = f_i - w_i (density
+ * rho = 1 + sum h_i; the equilibrium is written for h), as the GPU kernel does;
+ * raw population reads add w back.
  *
  * What it restates (reference paths relative to the repository root):
  *   - direction tables: BoltzmannFluids/Source/LBMConstants/VelocitySet.cpp:53-141,180-203
@@ -18,7 +21,7 @@
  *   - AA ("esoteric pull") read/write addressing: collide.comp:244-369
  *     (even tick: read slot i at x-c_i, write slot opp(i) at x+c_i;
  *      odd tick: read slot opp(i) at x, write slot i at x;
- *      non-periodic out-of-range read -> 0.01, write dropped: :297-298,:358-359)
+ *      non-periodic out-of-range read -> f = 0.01, write dropped: :297-298,:358-359)
  *   - the fused tick: collide.comp:676-793 (solid skip :696-700, density floor
  *     :705, force :707-720, half-force velocity shift and clamps :722-725,
  *     BGK + Guo source :727-745, Shan-Chen :534-584, thermal :771-782,
@@ -230,26 +233,16 @@ static int nbr(const orc_solver *s, int x, int y, int z, const int *c, int sgn, 
     return 1;
 }
 
-/* Storage round trip of one population (shifted 16-bit in the FP32 build). */
-static REAL store_val(const orc_solver *s, int i, REAL v) {
+/* Storage of one shifted population h (rounded to FP16/BF16 in the FP32 build). */
+static REAL store_val(const orc_solver *s, int i, REAL h) {
+    (void)i;
 #ifdef ORACLE_F32
-    float w = (float)s->T[i].w;
-    if (s->storage == 0) return (REAL)(half_rne((float)v - w));
-    if (s->storage == 2) return (REAL)(bf16_rne((float)v - w));
-    return v;
+    if (s->storage == 0) return (REAL)half_rne((float)h);
+    if (s->storage == 2) return (REAL)bf16_rne((float)h);
+    return h;
 #else
-    (void)s; (void)i;
-    return v;
-#endif
-}
-static REAL load_val(const orc_solver *s, int i, REAL raw) {
-#ifdef ORACLE_F32
-    float w = (float)s->T[i].w;
-    if (s->storage == 0 || s->storage == 2) return (REAL)((float)raw + w);
-    return raw;
-#else
-    (void)s; (void)i;
-    return raw;
+    (void)s;
+    return h;
 #endif
 }
 
@@ -258,12 +251,12 @@ static REAL load_val(const orc_solver *s, int i, REAL raw) {
 
 static int is_solid(const orc_solver *s, long c) { return s->has_ids && s->ids[c] != 0; }
 
-/* Equilibrium, second order, c_s^2 = 1/3 (collide.comp:589-606 with exact 3, 9/2, 3/2
- * as in set_equilibrium_populations.comp:179-196):
- *   w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2) = (w rho) (c1 + cu (3 + 9/2 cu)),
- * with c1 = 1 - 3/2 u^2 computed once per cell. */
-static REAL feq(REAL w, REAL rho, REAL cu, REAL c1) {
-    return (w * rho) * FMA(cu, FMA((REAL)4.5, cu, (REAL)3), c1);
+/* Shifted equilibrium h_eq = f_eq - w, second order, c_s^2 = 1/3 (collide.comp:589-606
+ * with exact 3, 9/2, 3/2 as in set_equilibrium_populations.comp:179-196):
+ *   w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2) - w = (w rho) (cu (3 + 9/2 cu) + c1) + w d,
+ * with c1 = -3/2 u^2 computed once per cell and d = rho - 1. */
+static REAL heq(REAL w, REAL rho, REAL d, REAL cu, REAL c1) {
+    return FMA(w * rho, FMA(cu, FMA((REAL)4.5, cu, (REAL)3), c1), w * d);
 }
 
 /* Thermal equilibrium with a configurable c_s^2 (collide.comp:634-651):
@@ -282,16 +275,16 @@ static REAL dot3(REAL ax, REAL ay, REAL az, REAL bx, REAL by, REAL bz) {
 static REAL read_pop(const orc_solver *s, int x, int y, int z, int i, int odd) {
     long self = cid(s, x, y, z);
     const int *c = s->T[i].c;
-    if (odd) return load_val(s, opp(i), F_AT(s, opp(i), self));
+    if (odd) return F_AT(s, opp(i), self);
     long src;
-    if (!nbr(s, x, y, z, c, -1, &src)) return (REAL)0.01;
+    if (!nbr(s, x, y, z, c, -1, &src)) return (REAL)0.01 - (REAL)s->T[i].w;   /* f = 0.01 */
     if (s->bb_mode == 0 && is_solid(s, src)) {
         const REAL *o = s->obj[s->ids[src]];
         REAL cu = (REAL)c[0] * o[0] + (REAL)c[1] * o[1] + (REAL)c[2] * o[2];
         /* f_i(x) = f*_opp(i)(x) - 2 w_opp rho_w (c_opp . u_w) * 3, rho_w = 1 (stream.comp:487-496,577-580) */
-        return FMA((REAL)6 * (REAL)s->T[i].w, cu, load_val(s, opp(i), F_AT(s, opp(i), self)));
+        return FMA((REAL)6 * (REAL)s->T[i].w, cu, F_AT(s, opp(i), self));
     }
-    return load_val(s, i, F_AT(s, i, src));
+    return F_AT(s, i, src);
 }
 
 /* AA write of post-collision population i of the cell at (x,y,z) (collide.comp:307-369). */
@@ -359,9 +352,10 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
             int x = (int)(cc % s->X), y = (int)((cc / s->X) % s->Y), z = (int)(cc / ((long)s->X * s->Y));
             long cell = cid(s, x, y, z);
             if (is_solid(s, cell)) continue;
-            REAL rho = 0;
-            for (int i = 0; i < s->q; ++i) rho += read_pop(s, x, y, z, i, odd);
-            if (rho < (REAL)0.01) rho = (REAL)0.01;
+            REAL d = 0;
+            for (int i = 0; i < s->q; ++i) d += read_pop(s, x, y, z, i, odd);
+            REAL rho = (REAL)1 + d;
+            if (!(rho >= (REAL)0.01)) rho = (REAL)0.01;
             psi_cur[cell] = psi_of(rho);
         }
         psi_prev = s->psi[s->cur];
@@ -377,15 +371,17 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
         long cell = cid(s, x, y, z);
         if (is_solid(s, cell)) continue;                 /* collide.comp:696-700 */
         for (int i = 0; i < s->q; ++i) f[i] = read_pop(s, x, y, z, i, odd);
-        REAL rho = 0, m[3] = {0, 0, 0};
-        for (int i = 0; i < s->q; ++i) rho += f[i];      /* collide.comp:383-388 */
+        /* d = rho - 1 = sum h_i; sum c_i w_i = 0, so the momentum sums h too */
+        REAL d = 0, m[3] = {0, 0, 0};
+        for (int i = 0; i < s->q; ++i) d += f[i];        /* collide.comp:383-388 */
+        REAL rho = (REAL)1 + d;
         for (int i = 0; i < s->q; ++i) {                 /* collide.comp:390-396 */
             m[0] += f[i] * (REAL)s->T[i].c[0];
             m[1] += f[i] * (REAL)s->T[i].c[1];
             m[2] += f[i] * (REAL)s->T[i].c[2];
         }
-        if (want_sums) { s0 += (double)rho; s1 += (double)m[0]; s2 += (double)m[1]; s3 += (double)m[2]; }
-        if (rho < (REAL)0.01) rho = (REAL)0.01;          /* collide.comp:705 */
+        if (want_sums) { s0 += 1.0 + (double)d; s1 += (double)m[0]; s2 += (double)m[1]; s3 += (double)m[2]; }
+        if (!(rho >= (REAL)0.01)) { rho = (REAL)0.01; d = (REAL)-0.99; }   /* collide.comp:705 */
         REAL F[3] = {0, 0, 0};
         if (s->forcing == 1) { F[0] = s->fconst[0]; F[1] = s->fconst[1]; F[2] = s->fconst[2]; }
         else if (s->forcing == 2) { F[0] = s->force[3 * cell]; F[1] = s->force[3 * cell + 1]; F[2] = s->force[3 * cell + 2]; }
@@ -422,7 +418,7 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
         if (update) {                                     /* collide.comp:727-745 */
             REAL uu = dot3(u[0], u[1], u[2], u[0], u[1], u[2]);
             REAL fu = dot3(F[0], F[1], F[2], u[0], u[1], u[2]);
-            REAL c1 = FMA((REAL)-1.5, uu, (REAL)1);
+            REAL c1 = (REAL)-1.5 * uu;
             REAL inv_tau = (REAL)1 / s->tau;
             REAL pre = (REAL)1 - (REAL)1 / ((REAL)2 * s->tau);
             int src_on = s->forcing || s->multiphase || s->thermal;
@@ -430,11 +426,11 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
                 const int *c = s->T[i].c;
                 REAL w = (REAL)s->T[i].w;
                 REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
-                REAL e = feq(w, rho, cu, c1);
+                REAL e = heq(w, rho, d, cu, c1);
                 REAL v;
                 if (ext) v = e;
                 else {
-                    v = FMA(e - f[i], inv_tau, f[i]);   /* f - (f - feq)/tau */
+                    v = FMA(e - f[i], inv_tau, f[i]);   /* h - (h - heq)/tau */
                     if (src_on) {
                         REAL cf = (REAL)c[0] * F[0] + (REAL)c[1] * F[1] + (REAL)c[2] * F[2];
                         /* Guo: (1 - 1/(2 tau)) w (3 cF + 9 cF cu - 3 F.u) */
@@ -597,11 +593,11 @@ ORC_API int orc_init_fields(orc_solver *s, const double *rho, const double *u, c
         REAL uu3[3] = {(REAL)u[3 * c], (REAL)u[3 * c + 1], (REAL)u[3 * c + 2]};
         if (s->has_ids && s->ids[c]) uu3[0] = uu3[1] = uu3[2] = 0;
         REAL uu = dot3(uu3[0], uu3[1], uu3[2], uu3[0], uu3[1], uu3[2]);
-        REAL c1 = FMA((REAL)-1.5, uu, (REAL)1);
+        REAL c1 = (REAL)-1.5 * uu;
         for (int i = 0; i < s->q; ++i) {
             const int *cc = s->T[i].c;
             REAL cu = (REAL)cc[0] * uu3[0] + (REAL)cc[1] * uu3[1] + (REAL)cc[2] * uu3[2];
-            F_AT(s, i, c) = store_val(s, i, feq((REAL)s->T[i].w, r, cu, c1));
+            F_AT(s, i, c) = store_val(s, i, heq((REAL)s->T[i].w, r, r - (REAL)1, cu, c1));
         }
     }
     free(s->g); s->g = NULL;
@@ -680,7 +676,7 @@ ORC_API int orc_read_field(const orc_solver *s, int field, double *dst) {
     case 2: for (long c = 0; c < 4 * N; ++c) dst[c] = s->ft[c]; return 0;
     case 3:
         for (int i = 0; i < s->q; ++i)
-            for (long c = 0; c < N; ++c) dst[(long)i * N + c] = (double)load_val(s, i, F_AT(s, i, c));
+            for (long c = 0; c < N; ++c) dst[(long)i * N + c] = (double)(F_AT(s, i, c) + (REAL)s->T[i].w);
         return 0;
     case 4:
         if (!s->thermal) return 1;
@@ -697,7 +693,8 @@ ORC_API int orc_read_field(const orc_solver *s, int field, double *dst) {
 /* Overwrite the raw lattice (AA slot layout, unshifted values) — test hook. */
 ORC_API void orc_write_populations(orc_solver *s, const double *src) {
     for (int i = 0; i < s->q; ++i)
-        for (long c = 0; c < s->N; ++c) F_AT(s, i, c) = store_val(s, i, (REAL)src[(long)i * s->N + c]);
+        for (long c = 0; c < s->N; ++c)
+            F_AT(s, i, c) = store_val(s, i, (REAL)src[(long)i * s->N + c] - (REAL)s->T[i].w);
 }
 
 /* AA address maps for the race/permutation tests: for cell c and direction i,

@@ -1,0 +1,119 @@
+"""Long runs of the benchmark scenarios at the sizes the FP64 oracle finishes
+in seconds: the GPU results against the FP64 CPU restatement after thousands
+of ticks, and the conserved sums."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from boltzmannfluids_amd.scenarios import lid_cavity, porous_spheres, spinodal
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_lid_driven_cavity_256_10000_ticks(bfa, orc):
+    """D2Q9 256^2 cavity, tau 0.6, lid 0.1, 10000 ticks: FP32 GPU vs FP64 oracle,
+    and the FP64 mass sums of both."""
+    n, ticks = 256, 10000
+    cells = n * n
+    ids = lid_cavity(n, n)
+    init = dict(rho=np.ones(cells, np.float32), u=np.zeros((cells, 3), np.float32), boundary_id=ids)
+    say(f"case: D2Q9 cavity {n}^2 {ticks} ticks")
+    g = bfa.Solver((n, n, 1), 0.6, periodic=(0, 0, 1), velocity_set="D2Q9")
+    g.set_object(2, velocity=(0.1, 0.0, 0.0))
+    g.init_fields(**init)
+    m0 = g.conservation()[0]
+    g.step(ticks, save=True)
+    t0 = time.time()
+    o = orc.Oracle((n, n, 1), 0.6, periodic=(0, 0, 1), velocity_set="D2Q9", precision="f64")
+    o.set_object(2, velocity=(0.1, 0.0, 0.0))
+    o.init_fields(**init)
+    o.step(ticks, save=True)
+    say(f"  oracle {time.time() - t0:.1f} s")
+    fluid = ids == 0
+    a, b = g.read("velocity_density")[fluid].astype(np.float64), o.read("velocity_density")[fluid]
+    er, eu = rel_l2(a[:, 3], b[:, 3]), rel_l2(a[:, :3], b[:, :3])
+    mg, mo = g.conservation()[0], o.conservation()[0]
+    say(f"  vs FP64: rho {er:.3e} u {eu:.3e}; mass GPU {mg:.12f} oracle {mo:.12f} start {m0:.12f}")
+    assert er < 1e-5 and eu < 1e-4
+    assert abs(mg - mo) / mo < 1e-6 and abs(mo - m0) / m0 < 1e-12
+    # the lid drives a primary vortex: u_x changes sign between top and bottom
+    ux = a[:, 0]
+    assert ux.max() > 0.03 and ux.min() < -0.01
+
+
+@pytest.mark.parametrize("storage,bar", [("fp32", 1e-4), ("fp16", 1e-3), ("bf16", 1e-2)])
+def test_porous_duct_2000_ticks(bfa, orc, storage, bar):
+    """D3Q19 porous duct (the 384^3 workload's generator at 64^3), g_x 1e-6,
+    2000 ticks: each storage type against the FP64 oracle on the fluid cells.
+    The mean |u| is ~1e-3 lattice units (density deviations ~1e-5), so the
+    populations h = f - w are ~1e-4 and FP32 rounding moves u by ~1e-8 of |u|
+    per tick; over 2000 ticks that adds up to a few 1e-5 relative.  FP16 and
+    BF16 storage are bounded by their quantum of h."""
+    n, ticks = 64, 2000
+    cells = n ** 3
+    ids = porous_spheres(n, n, n)
+    force = np.tile(np.array([1e-6, 0, 0], np.float32), (cells, 1))
+    init = dict(rho=np.ones(cells, np.float32), u=np.zeros((cells, 3), np.float32), force=force, boundary_id=ids)
+    say(f"case: porous {n}^3 {storage} {ticks} ticks")
+    g = bfa.Solver((n, n, n), 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", storage=storage)
+    g.init_fields(**init)
+    g.step(ticks, save=True)
+    o = orc.Oracle((n, n, n), 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", precision="f64")
+    o.init_fields(**init)
+    o.step(ticks, save=True)
+    fluid = ids == 0
+    a, b = g.read("velocity_density")[fluid].astype(np.float64), o.read("velocity_density")[fluid]
+    er, eu = rel_l2(a[:, 3], b[:, 3]), rel_l2(a[:, :3], b[:, :3])
+    say(f"  vs FP64: rho {er:.3e} u {eu:.3e}")
+    assert er < bar and eu < bar
+    # forcing accelerates the flow along +x: the mean fluid u_x is positive
+    assert b[:, 0].mean() > 0 and a[:, 0].mean() > 0
+
+
+def test_shan_chen_thermal_2000_ticks(bfa, orc):
+    """The D3Q27 Shan-Chen + D3Q7 thermal workload at 32^3, 2000 ticks: FP64
+    mass sums stay constant on the GPU, and the GPU FP32 fields follow the FP32
+    oracle (the pseudopotential's exp differs in the last bit, so the two
+    trajectories agree to rounding growth, not bit for bit)."""
+    n, ticks = 32, 2000
+    cells = n ** 3
+    rho, u, ids = spinodal(n, n, n)
+    kw = dict(force=np.tile(np.array([0, -1e-5, 0], np.float32), (cells, 1)),
+              temperature=np.full(cells, 0.5, np.float32), boundary_id=ids)
+    say(f"case: Shan-Chen + thermal {n}^3 {ticks} ticks")
+    runs = []
+    for mk in (lambda: bfa.Solver((n, n, n), 1.0, periodic=(1, 0, 1), velocity_set="D3Q27", multiphase=True),
+               lambda: orc.Oracle((n, n, n), 1.0, periodic=(1, 0, 1), velocity_set="D3Q27", multiphase=True,
+                                  precision="f32")):
+        s = mk()
+        s.set_object(1, temperature=1.0)
+        s.set_object(2, temperature=0.0)
+        s.set_param("interaction_strength", -5.0)
+        s.set_param("reference_temperature", 0.5)
+        s.init_fields(rho, u, **kw)
+        m0 = s.conservation()[0]
+        s.step(ticks, save=True)
+        runs.append((s, m0, s.conservation()[0]))
+    (g, gm0, gm1), (o, om0, om1) = runs
+    say(f"  mass drift GPU {abs(gm1 - gm0) / gm0:.3e} oracle {abs(om1 - om0) / om0:.3e}")
+    # the liquid holds rho ~ 2 (h is not small there), so FP32 rounding drifts
+    # the mass by a few 1e-6 over 2000 ticks, on the GPU as in the FP32 oracle
+    assert abs(gm1 - gm0) / gm0 < 1e-5
+    a, b = g.read("velocity_density").astype(np.float64), o.read("velocity_density")
+    ta, tb = g.read("force_temperature")[:, 3].astype(np.float64), o.read("force_temperature")[:, 3]
+    fluid = ids == 0
+    er, et = rel_l2(a[fluid, 3], b[fluid, 3]), rel_l2(ta[fluid], tb[fluid])
+    say(f"  vs FP32 oracle: rho {er:.3e} T {et:.3e}")
+    assert er < 1e-3 and et < 1e-3
+    # heat flows from the hot wall (y = 0) to the cold one
+    T = ta.reshape(n, n, n)
+    assert T[:, 1, :].mean() > 0.5 > T[:, n - 2, :].mean()
