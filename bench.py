@@ -289,6 +289,20 @@ def counters_sample(bfa, n=256, ticks=10):
     print("counters sample done", flush=True)
 
 
+def adjoint_sample(bfa, ticks):
+    """Short workload for rocprofv3: the D3Q19 128^3 channel forward and its
+    gradient over `ticks` ticks with checkpoints only (K = 100)."""
+    import torch  # noqa: F401  (one HIP runtime with torch)
+    res = (128, 128, 128)
+    ch = bfa.capi.Channel(res, 0.6, 0.05, velocity_set="D3Q19")
+    ch.set_porosity(porosity(*res))
+    ch.forward(ticks)
+    ch.gradient(ticks, checkpoint_every=100, trajectory_budget_bytes=0)
+    t = ch.timing()
+    ch.close()
+    print(f"adjoint sample: {ticks} ticks, gradient {t[2]:.1f} ms (recompute {t[1]:.1f} ms)", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,6 +316,8 @@ def main():
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--counters-sample", action="store_true",
                     help="run only the short workload used for rocprofv3 --pmc passes")
+    ap.add_argument("--adjoint-sample", type=int, default=0, metavar="TICKS",
+                    help="run only the D3Q19 128^3 channel gradient over TICKS ticks (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solver runs on a single MI355X; --gpus {args.gpus} is not supported",
@@ -311,6 +327,9 @@ def main():
     bfa.load_library()
     if args.counters_sample:
         counters_sample(bfa, args.size)
+        return
+    if args.adjoint_sample:
+        adjoint_sample(bfa, args.adjoint_sample)
         return
     n = args.size
     stores = ["fp32"] if args.fp32_only else ["fp32", "fp16", "bf16"]

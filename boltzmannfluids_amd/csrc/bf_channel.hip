@@ -18,10 +18,13 @@
 // direction), closes the boundaries, collides and writes s^t locally.  The
 // adjoint of a tick runs per cell: it reads dJ/ds^t at the cell, re-derives the
 // cell's forward state from s^{t-1}, applies the transpose of collision +
-// partial bounce-back (O(Q) through three moment sums), the transpose of the
-// boundary closure, and scatters to the Q slots of s^{t-1} the cell pulled from.
-// Every pulled slot has exactly one puller, so the scatter is race free and
-// needs plain stores.
+// partial bounce-back (O(Q) through three moment sums) and the transpose of the
+// boundary closure, giving m_i(x) = dJ/d(pulled f_i at x).  m is stored at the
+// puller (slot i of cell x: aligned, local stores); the next adjoint tick
+// gathers dJ/ds^{t-1}_i(y) = m_i(y + c_i) with the mirror image of the forward
+// pull (zero where no cell pulls the slot, m_opp(i)(y) where y pulled its own
+// slot back from a wall).  Every slot has at most one puller, so the gather is
+// exact, and every tick moves its misaligned accesses into loads.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -185,9 +188,32 @@ __global__ void __launch_bounds__(256) k_ch_step(const ChArgs a, const float *__
         stnt(dst + (size_t)i * a.stride + p.self, (1.0f - g) * fs[i] + g * f[opposite(i)]);
 }
 
+// dJ/ds_i(y) from the puller-indexed adjoint m of the following tick: the
+// forward pull (pull above) read slot i of cell y from cell y + c_i, or from y
+// itself through direction opp(i) when y + c_i lies in a wall row, or not at
+// all when y + c_i leaves the channel in x.
 template <int VS>
+__device__ inline void gather(const ChArgs &a, const float *__restrict__ m, const CellPos &p, float *mu) {
+    using L = Lattice<VS>;
+    const int XY = a.X * a.Y;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const int tx = p.x + L::c[i][0], ty = p.y + L::c[i][1];
+        const int tz = L::c[i][2] > 0 ? p.zp : (L::c[i][2] < 0 ? p.zm : p.z);
+        if (L::c[i][0] != 0 && (tx < 0 || tx >= a.X)) { mu[i] = 0.0f; continue; }
+        if (L::c[i][1] != 0 && (ty == 0 || ty == a.Y - 1)) {
+            mu[i] = ldnt(m + (size_t)opposite(i) * a.stride + p.self);
+            continue;
+        }
+        mu[i] = ldnt(m + (size_t)i * a.stride + tx + ty * a.X + tz * XY);
+    }
+}
+
+// One adjoint tick.  PLAIN: `nu` holds dJ/ds^t slot by slot (the seed);
+// otherwise `nu` is the puller-indexed m of tick t + 1.  Writes m of tick t.
+template <int VS, bool PLAIN>
 __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float *__restrict__ src,
-                                                    const float *__restrict__ nu, float *__restrict__ nu_prev,
+                                                    const float *__restrict__ nu, float *__restrict__ m_out,
                                                     float *__restrict__ dg, double *__restrict__ du_rows) {
     using L = Lattice<VS>;
     constexpr int Q = L::Q;
@@ -196,8 +222,12 @@ __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float 
     // the adjoint slots are loaded before the state is pulled: both sets of
     // loads are in flight together
     float mu[Q];
+    if constexpr (PLAIN) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) mu[i] = ldnt(nu + (size_t)i * a.stride + p.self);
+        for (int i = 0; i < Q; ++i) mu[i] = ldnt(nu + (size_t)i * a.stride + p.self);
+    } else {
+        gather<VS>(a, nu, p, mu);
+    }
     const float g = a.gamma[p.self];
     float f[Q];
     int sign;
@@ -270,19 +300,9 @@ __global__ void __launch_bounds__(256) k_ch_adjoint(const ChArgs a, const float 
         }
     }
 
-    // transpose of the pull
-    const int XY = a.X * a.Y;
+    // m at the puller; the transpose of the pull is the next tick's gather
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const int sx = p.x - L::c[i][0], sy = p.y - L::c[i][1];
-        const int sz = L::c[i][2] > 0 ? p.zm : (L::c[i][2] < 0 ? p.zp : p.z);
-        if (L::c[i][0] != 0 && (sx < 0 || sx >= a.X)) continue;
-        if (L::c[i][1] != 0 && (sy == 0 || sy == a.Y - 1)) {
-            stnt(nu_prev + (size_t)opposite(i) * a.stride + p.self, mh[i]);
-            continue;
-        }
-        stnt(nu_prev + (size_t)i * a.stride + sx + sy * a.X + sz * XY, mh[i]);
-    }
+    for (int i = 0; i < Q; ++i) stnt(m_out + (size_t)i * a.stride + p.self, mh[i]);
 }
 
 // J = mean over the outlet cells of u_x = (S / rho_out - 1); one block, fixed order.
@@ -459,13 +479,18 @@ hipError_t launch_tick(const bf_channel *c, const float *src, float *dst) {
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(const bf_channel *c, const float *src, const float *nu, float *nu_prev, float *dg,
+hipError_t launch_adjoint(const bf_channel *c, bool plain, const float *src, const float *nu, float *m_out, float *dg,
                           double *du_rows) {
     dim3 g, b;
     shape(c, g, b);
     const ChArgs a = args_of(c);
-    if (c->q == 9) hipLaunchKernelGGL(k_ch_adjoint<D2Q9>, g, b, 0, c->stream, a, src, nu, nu_prev, dg, du_rows);
-    else hipLaunchKernelGGL(k_ch_adjoint<D3Q19>, g, b, 0, c->stream, a, src, nu, nu_prev, dg, du_rows);
+    if (c->q == 9) {
+        if (plain) hipLaunchKernelGGL((k_ch_adjoint<D2Q9, true>), g, b, 0, c->stream, a, src, nu, m_out, dg, du_rows);
+        else hipLaunchKernelGGL((k_ch_adjoint<D2Q9, false>), g, b, 0, c->stream, a, src, nu, m_out, dg, du_rows);
+    } else {
+        if (plain) hipLaunchKernelGGL((k_ch_adjoint<D3Q19, true>), g, b, 0, c->stream, a, src, nu, m_out, dg, du_rows);
+        else hipLaunchKernelGGL((k_ch_adjoint<D3Q19, false>), g, b, 0, c->stream, a, src, nu, m_out, dg, du_rows);
+    }
     return hipGetLastError();
 }
 
@@ -688,7 +713,8 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
         else hipLaunchKernelGGL(k_ch_seed<D3Q19>, g, b, 0, c->stream, a, nu[0]);
         CH_HIP(c, hipGetLastError());
     }
-    int cur = 0;   // nu[cur] = dJ/ds^t
+    int cur = 0;          // nu[cur]: dJ/ds^T (the seed), then m of the tick after
+    bool plain = true;    // only the first adjoint tick reads the seed slot by slot
     double recompute_ms = 0.0;
     for (int64_t k = nseg - 1; k >= 0; --k) {
         const int64_t t0 = k * K, len = std::min(K, T - t0);
@@ -703,8 +729,9 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
             recompute_ms += ms;
         }
         for (int64_t j = len - 1; j >= 0; --j) {
-            CH_HIP(c, launch_adjoint(c, seg_state(k, j), nu[cur], nu[cur ^ 1], dgseg, du_rows));
+            CH_HIP(c, launch_adjoint(c, plain, seg_state(k, j), nu[cur], nu[cur ^ 1], dgseg, du_rows));
             cur ^= 1;
+            plain = false;
         }
         hipLaunchKernelGGL(k_fold, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, dgseg, dg64, c->N);
         CH_HIP(c, hipGetLastError());
