@@ -76,6 +76,23 @@ template <class T>
 __device__ inline T *at(T *p, uint32_t idx) {
     return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + idx * (uint32_t)sizeof(T));
 }
+// Byte offset of element idx with its known bits hidden.  Where one index
+// serves every slot (the odd tick), the compiler otherwise rewrites
+// zext(idx * size) as zext(idx) << log2(size), a 64-bit offset that the
+// scalar-base form cannot take, and builds a 64-bit address per access.
+__device__ inline uint32_t opaque_offset(uint32_t idx, uint32_t size) {
+    uint32_t off = idx * size;
+    asm("" : "+v"(off));
+    return off;
+}
+template <class T>
+__device__ inline T *at_bytes(T *p, uint32_t off) {
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + off);
+}
+template <class T>
+__device__ inline const T *at_bytes(const T *p, uint32_t off) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + off);
+}
 
 // Storage policies.  Every storage type holds h = f - w (the population minus
 // its rest-state weight), and the tick computes on h: at low Mach number f is
@@ -90,18 +107,24 @@ template <> struct Store<ST_FP32> {
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v) { return v; }
     __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), h); }
+    __device__ static T raw_b(const T *p, uint32_t off) { return ld(at_bytes(p, off)); }
+    __device__ static void store_b(T *p, uint32_t off, float h) { st(at_bytes(p, off), h); }
 };
 template <> struct Store<ST_FP16> {
     using T = _Float16;
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v) { return (float)v; }
     __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), (_Float16)h); }
+    __device__ static T raw_b(const T *p, uint32_t off) { return ld(at_bytes(p, off)); }
+    __device__ static void store_b(T *p, uint32_t off, float h) { st(at_bytes(p, off), (_Float16)h); }
 };
 template <> struct Store<ST_BF16> {
     using T = uint16_t;
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v) { return __uint_as_float((uint32_t)v << 16); }
     __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), bf16_rne(h)); }
+    __device__ static T raw_b(const T *p, uint32_t off) { return ld(at_bytes(p, off)); }
+    __device__ static void store_b(T *p, uint32_t off, float h) { st(at_bytes(p, off), bf16_rne(h)); }
 };
 
 // c_i . v summed over the non-zero components of c_i in x, y, z order.  Equal
@@ -280,8 +303,12 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int o = opposite(i);
-            if (ODD || i == 0) {
-                raw[k][i] = S::raw(SLOT((ODD ? o : 0)), act ? self[k] : 0);
+            if constexpr (ODD) {
+                raw[k][i] = S::raw_b(SLOT(o), opaque_offset(act ? self[k] : 0, sizeof(T)));
+                continue;
+            }
+            if (i == 0) {
+                raw[k][i] = S::raw(SLOT(0), act ? self[k] : 0);
                 continue;
             }
             const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
@@ -300,7 +327,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int q = 0; q < QT; ++q) {
                 const int o = opposite(q);
                 if (ODD || q == 0) {
-                    graw[q] = *at(a.gslot[(ODD ? o : 0)], act ? self[k] : 0);
+                    if constexpr (ODD) graw[q] = *at_bytes(a.gslot[o], opaque_offset(act ? self[k] : 0, 4));
+                    else graw[q] = *at(a.gslot[0], act ? self[k] : 0);
                     continue;
                 }
                 const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
@@ -525,7 +553,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                         v += (pre * w) * (__builtin_fmaf(9.0f * cf, cu, 3.0f * cf) - 3.0f * fu);
                     }
                 }
-                if (ODD || i == 0) {
+                if (ODD) {
+                    S::store_b(SLOT(i), opaque_offset(self[k], sizeof(T)), v);
+                } else if (i == 0) {
                     S::store(SLOT(i), self[k], v);
                 } else {
                     const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
@@ -565,7 +595,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     // wt T (1 + cu/cs2 + cu^2/(2 cs2^2) - u^2/(2 cs2))
                     const float e = (L::wt[q] * temp) * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
                     const float v = __builtin_fmaf(e - gi[q], inv_tau_t, gi[q]);
-                    if (ODD || q == 0) {
+                    if (ODD) {
+                        *at_bytes(a.gslot[q], opaque_offset(self[k], 4)) = v;
+                    } else if (q == 0) {
                         *at(a.gslot[q], self[k]) = v;
                     } else {
                         const int dx = 1 + L::c[q][0], dy = 1 + L::c[q][1], dz = 1 + L::c[q][2];

@@ -105,6 +105,27 @@ def test_velocity_sets_and_storage(bfa, orc, vs, storage):
     assert_macro_vs_f64(g, o64, {"fp32": 5e-5, "fp16": 1e-3, "bf16": 1e-2}[storage])
 
 
+@pytest.mark.parametrize("vs,res", [("D2Q9", (128, 20, 1)), ("D3Q15", (64, 6, 5)), ("D3Q19", (320, 4, 3)),
+                                    ("D3Q27", (64, 5, 4))])
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_periodic_rows_multiple_of_64(bfa, orc, vs, res, storage):
+    """Fully periodic grids whose rows fill whole waves: the even tick's row
+    exchange with the periodic x wrap at X = 64 (one wave per row, several
+    rows per block), 128 (two waves per row) and 320 (five waves per row);
+    11 ticks end on an even tick."""
+    n = int(np.prod(res))
+    rho, u = random_state(n, seed=SEED + 7, du=0.06, drho=0.04)
+    g, o = run_pair(bfa, orc, res, 0.6, 11, velocity_set=vs, storage=storage, init=dict(rho=rho, u=u))
+    a, b = g.read("populations_raw").astype(np.float64), o.read("populations_raw")
+    same = float(np.mean(a == b))
+    say(f"  raw max|diff|={np.abs(a - b).max():.3e} identical={same:.4f}")
+    if storage == "fp32":
+        assert same == 1.0
+        np.testing.assert_array_equal(g.read("velocity_density"), o.read("velocity_density").astype(np.float32))
+    else:
+        assert same > 0.98
+
+
 @pytest.mark.parametrize("mode", ["halfway", "fullway_compat"])
 def test_lid_driven_cavity_walls(bfa, orc, mode):
     X = Y = 48
