@@ -289,6 +289,22 @@ def counters_sample(bfa, n=256, ticks=10):
     print("counters sample done", flush=True)
 
 
+def porous_sample(bfa, ticks, n=384):
+    """Short workload for rocprofv3: `ticks` ticks of the D3Q19 porous duct
+    (constant g_x) in FP32, FP16 and BF16."""
+    import torch  # noqa: F401  (one HIP runtime with torch)
+    ids = porous_spheres(n, n, n)
+    cells = n ** 3
+    for st in ("fp32", "fp16", "bf16"):
+        s = bfa.Solver((n, n, n), 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", storage=st)
+        s.init_fields(np.ones(cells, np.float32), np.zeros((cells, 3), np.float32), boundary_id=ids)
+        s.set_param("constant_force", np.array([1e-6, 0, 0], np.float32))
+        s.step(ticks)
+        s.synchronize()
+        s.close()
+    print(f"porous sample: {n}^3, {ticks} ticks per storage type", flush=True)
+
+
 def adjoint_sample(bfa, ticks):
     """Short workload for rocprofv3: the D3Q19 128^3 channel forward and its
     gradient over `ticks` ticks with checkpoints only (K = 100)."""
@@ -316,6 +332,8 @@ def main():
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--counters-sample", action="store_true",
                     help="run only the short workload used for rocprofv3 --pmc passes")
+    ap.add_argument("--porous-sample", type=int, default=0, metavar="TICKS",
+                    help="run only TICKS ticks of the 384^3 porous duct per storage type (for rocprofv3)")
     ap.add_argument("--adjoint-sample", type=int, default=0, metavar="TICKS",
                     help="run only the D3Q19 128^3 channel gradient over TICKS ticks (for rocprofv3)")
     args = ap.parse_args()
@@ -327,6 +345,9 @@ def main():
     bfa.load_library()
     if args.counters_sample:
         counters_sample(bfa, args.size)
+        return
+    if args.porous_sample:
+        porous_sample(bfa, args.porous_sample)
         return
     if args.adjoint_sample:
         adjoint_sample(bfa, args.adjoint_sample)

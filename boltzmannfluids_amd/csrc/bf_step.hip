@@ -101,9 +101,19 @@ __device__ inline const T *at_bytes(const T *p, uint32_t off) {
 // that w occupies.  `raw` issues the load, `cvt` widens it to FP32; the tick
 // issues all raw loads of a cell before it converts any, so the loads stay in
 // flight together.
+//
+// Opposite slots i and opp(i) (adjacent: odd i pairs with i + 1) are addressed
+// as a pair from the lower slot's base: element e of the pair is cell e of the
+// lower slot for e < stride and cell e - stride of the higher one.  A per-lane
+// select of the element then replaces a per-lane select between two base
+// pointers.  Two 16-bit slots stay below 4 GiB for every grid the solver
+// accepts, so their byte offset is 32-bit (scalar-base form); FP32 pairs use a
+// 64-bit address.
 template <int ST> struct Store;
 template <> struct Store<ST_FP32> {
     using T = float;
+    __device__ static T raw_pair(const T *p, uint32_t e) { return ld(p + (size_t)e); }
+    __device__ static void store_pair(T *p, uint32_t e, float h) { st(p + (size_t)e, h); }
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v) { return v; }
     __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), h); }
@@ -112,6 +122,8 @@ template <> struct Store<ST_FP32> {
 };
 template <> struct Store<ST_FP16> {
     using T = _Float16;
+    __device__ static T raw_pair(const T *p, uint32_t e) { return ld(at(p, e)); }
+    __device__ static void store_pair(T *p, uint32_t e, float h) { st(at(p, e), (_Float16)h); }
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v) { return (float)v; }
     __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), (_Float16)h); }
@@ -120,6 +132,8 @@ template <> struct Store<ST_FP16> {
 };
 template <> struct Store<ST_BF16> {
     using T = uint16_t;
+    __device__ static T raw_pair(const T *p, uint32_t e) { return ld(at(p, e)); }
+    __device__ static void store_pair(T *p, uint32_t e, float h) { st(at(p, e), bf16_rne(h)); }
     __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
     __device__ static float cvt(T v) { return __uint_as_float((uint32_t)v << 16); }
     __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), bf16_rne(h)); }
@@ -198,6 +212,10 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 
     // slot bases come from the argument block (wave-uniform, scalar registers)
 #define SLOT(i) static_cast<T *>(a.slot[i])
+    // element of cell c of slot j in the pair (j, opp(j)) based at the lower slot
+    const uint32_t stride32 = (uint32_t)a.stride;
+#define PAIR_E(j, c) ((uint32_t)(c) + ((j) > opposite(j) ? stride32 : 0u))
+#define PAIR_BASE(j) SLOT((j) < opposite(j) ? (j) : opposite(j))
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int y = blockIdx.y * blockDim.y + ty;
@@ -316,10 +334,16 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             const bool redir = a.halfway && SOLID(k, o);
             if (AL && L::c[i][0] != 0) {
                 // own opposite slot for a wall redirect: loaded below
+            } else if constexpr (WALLS) {
+                // bounce-back reads slot o at the cell itself, otherwise slot i
+                // at the source: one pair, one base
+                const uint32_t e = !act ? 0u
+                                   : redir ? PAIR_E(o, self[k])
+                                           : PAIR_E(i, valid ? xs[k][dx] + ry[dy] + pz[dz] : self[k]);
+                raw[k][i] = S::raw_pair(PAIR_BASE(i), e);
             } else {
-                const int idx = !act ? 0 : (valid && !redir) ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
-                T *const pi = SLOT(i), *const po = SLOT(o);
-                raw[k][i] = S::raw(redir ? po : pi, idx);
+                const int idx = !act ? 0 : valid ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
+                raw[k][i] = S::raw(SLOT(i), idx);
             }
         }
         if constexpr (TH) {
@@ -333,8 +357,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 }
                 const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
                 const bool pull = OK(k, dx, dy, dz) && !SOLID(k, o);
-                const int idx = !act ? 0 : pull ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
-                graw[q] = *at(pull ? a.gslot[q] : a.gslot[o], idx);
+                const uint32_t e = !act ? 0u : pull ? PAIR_E(q, xs[k][dx] + ry[dy] + pz[dz]) : PAIR_E(o, self[k]);
+                graw[q] = a.gslot[q < o ? q : o][e];
             }
         }
     }
@@ -367,22 +391,20 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
             const bool valid = vx[k][dx] && vy[dy] && vz[dz];
             const bool redir = a.halfway && SOLID(k, o);
-            if (!valid) {
-                fi[k][i] = 0.01f - w;  // open boundary read f = 0.01 (collide.comp:297-298)
-            } else if (redir) {
-                // moving wall (Ladd): + 6 w rho_w (c_i . u_w), rho_w = 1; a wall at
-                // rest adds exactly zero.  (Aligned x-directions: after the row exchange.)
-                if (!(AL && L::c[i][0] != 0)) {
-                    float v = S::cvt(raw[k][i]);
-                    if (a.wall_ids) {
-                        const ObjectEntry ob = WALL(k, o);
-                        v = __builtin_fmaf(6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
-                    }
-                    fi[k][i] = v;
-                }
-            } else if (!(AL && L::c[i][0] != 0)) {
-                fi[k][i] = S::cvt(raw[k][i]);
+            if (AL && L::c[i][0] != 0) {
+                // the open-boundary value, replaced after the row exchange
+                // unless the source is outside the domain
+                fi[k][i] = 0.01f - w;
+                continue;
             }
+            float v = S::cvt(raw[k][i]);
+            if (a.wall_ids && redir) {
+                // moving wall (Ladd): + 6 w rho_w (c_i . u_w), rho_w = 1; a wall at
+                // rest adds exactly zero
+                const ObjectEntry ob = WALL(k, o);
+                v = __builtin_fmaf(6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
+            }
+            fi[k][i] = valid ? v : 0.01f - w;  // open boundary read f = 0.01 (collide.comp:297-298)
         }
         if constexpr (TH) {
 #pragma unroll
@@ -404,23 +426,24 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
                 const int dx = 1 - L::c[i][0], dy = 1 - L::c[i][1], dz = 1 - L::c[i][2];
-                if (!OK(k, dx, dy, dz)) continue;
-                if (a.halfway && SOLID(k, opposite(i))) continue;
-                fi[k][i] = LDS(i, xs[k][dx]);
+                const bool take = OK(k, dx, dy, dz) && !(a.halfway && SOLID(k, opposite(i)));
+                const float lv = LDS(i, xs[k][dx]);
+                fi[k][i] = take ? lv : fi[k][i];
             }
         }
         __syncthreads();
-        // every entry a cell of the block may write starts out empty
+        // Each row-buffer entry (i, col) has one writer, the cell at col - c_x
+        // (periodic in x within the row): cells that update write theirs in
+        // phase 2 (empty where the write is dropped or bounced back), the
+        // other cells of the grid mark theirs empty here.
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int x = xs[k][1];
-            if (x >= X || y >= Y) continue;
+            if (x >= X || y >= Y || active[k]) continue;
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
-                const int dx = 1 + L::c[i][0];
-                if (!(PER || vx[k][dx])) continue;
-                LDS(i, xs[k][dx]) = __uint_as_float(kNoValue);
+                LDS(i, xs[k][1 + L::c[i][0]]) = __uint_as_float(kNoValue);
             }
         }
         if constexpr (WALLS) {
@@ -441,13 +464,13 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 for (int i = 1; i < Q; ++i) {
                     if (L::c[i][0] == 0) continue;
                     const int o = opposite(i);
-                    if (!(active[k] && a.halfway && SOLID(k, o))) continue;
+                    const bool red = active[k] && a.halfway && SOLID(k, o);
                     float v = S::cvt(rr[i]);
-                    if (a.wall_ids) {
+                    if (a.wall_ids && red) {
                         const ObjectEntry ob = WALL(k, o);
                         v = __builtin_fmaf(6.0f * L::w[i], dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
                     }
-                    fi[k][i] = v;
+                    fi[k][i] = red ? v : fi[k][i];
                 }
             }
         }
@@ -557,22 +580,34 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     S::store_b(SLOT(i), opaque_offset(self[k], sizeof(T)), v);
                 } else if (i == 0) {
                     S::store(SLOT(i), self[k], v);
-                } else {
+                } else if constexpr (WALLS) {
+                    // the destination is slot o at x + c_i, or slot i at the cell
+                    // itself when x + c_i is solid (the address this cell read
+                    // f_o from); writes leaving the domain are dropped
+                    // (collide.comp:358-359)
                     const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
-                    if (!OK(k, dx, dy, dz)) {
-                        // open boundary: write dropped (collide.comp:358-359)
-                    } else if (a.halfway && SOLID(k, i)) {
-                        float vb = v;
-                        if (a.wall_ids) {
-                            const ObjectEntry ob = WALL(k, i);
-                            vb = __builtin_fmaf(-6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
-                        }
-                        S::store(SLOT(i), self[k], vb);
-                    } else if (AL && L::c[i][0] != 0) {
-                        LDS(i, xs[k][dx]) = v;
-                    } else {
-                        S::store(SLOT(o), xs[k][dx] + ry[dy] + pz[dz], v);
+                    const bool ok = OK(k, dx, dy, dz);
+                    const bool red = a.halfway && SOLID(k, i);
+                    float vb = v;
+                    if (a.wall_ids && red) {
+                        const ObjectEntry ob = WALL(k, i);
+                        vb = __builtin_fmaf(-6.0f * w, dotc<BF_VS>(i, ob.ux, ob.uy, ob.uz), v);
                     }
+                    if (AL && L::c[i][0] != 0) {
+                        // the row buffer entry has this cell as its only writer
+                        LDS(i, xs[k][dx]) = ok && !red ? v : __uint_as_float(kNoValue);
+                        if (ok && red) S::store(SLOT(i), self[k], vb);
+                    } else {
+                        const uint32_t e = red ? PAIR_E(i, self[k]) : PAIR_E(o, xs[k][dx] + ry[dy] + pz[dz]);
+                        if (ok) S::store_pair(PAIR_BASE(i), e, red ? vb : v);
+                    }
+                } else {
+                    // no walls: slot o at x + c_i; writes leaving the domain are
+                    // dropped (collide.comp:358-359)
+                    const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
+                    const bool ok = OK(k, dx, dy, dz);
+                    if (AL && L::c[i][0] != 0) LDS(i, xs[k][dx]) = ok ? v : __uint_as_float(kNoValue);
+                    else if (ok) S::store(SLOT(o), xs[k][dx] + ry[dy] + pz[dz], v);
                 }
             }
         }
@@ -600,15 +635,14 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     } else if (q == 0) {
                         *at(a.gslot[q], self[k]) = v;
                     } else {
+                        // zero-flux bounce-back at an open edge and anti-bounce-back
+                        // at a wall stay at the cell; otherwise slot o at x + c_q
                         const int dx = 1 + L::c[q][0], dy = 1 + L::c[q][1], dz = 1 + L::c[q][2];
-                        if (!OK(k, dx, dy, dz)) {
-                            *at(a.gslot[q], self[k]) = v;
-                        } else if (SOLID(k, q)) {
-                            *at(a.gslot[q], self[k]) =
-                                __builtin_fmaf(2.0f * L::wt[q], WALL(k, q).temperature, -v);
-                        } else {
-                            *at(a.gslot[o], xs[k][dx] + ry[dy] + pz[dz]) = v;
-                        }
+                        const bool ok = OK(k, dx, dy, dz), sol = SOLID(k, q);
+                        float vv = v;
+                        if (ok && sol) vv = __builtin_fmaf(2.0f * L::wt[q], WALL(k, q).temperature, -v);
+                        const uint32_t e = (!ok || sol) ? PAIR_E(q, self[k]) : PAIR_E(o, xs[k][dx] + ry[dy] + pz[dz]);
+                        a.gslot[q < o ? q : o][e] = vv;
                     }
                 }
             }
@@ -622,10 +656,11 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     if constexpr (AL) {
         __syncthreads();
         // store the values the row buffer received, each at its own column
+        // (a tick that does not update wrote no entries)
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int x = xs[k][1];
-            if (x >= X || y >= Y) continue;
+            if (x >= X || y >= Y || !a.update) continue;
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
@@ -640,6 +675,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     }
 
 #undef LDS
+#undef PAIR_E
+#undef PAIR_BASE
 #undef SLOT
 #undef OK
 #undef SOLID
