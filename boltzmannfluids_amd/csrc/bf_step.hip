@@ -336,14 +336,17 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 // own opposite slot for a wall redirect: loaded below
             } else if constexpr (WALLS) {
                 // bounce-back reads slot o at the cell itself, otherwise slot i
-                // at the source: one pair, one base
-                const uint32_t e = !act ? 0u
-                                   : redir ? PAIR_E(o, self[k])
-                                           : PAIR_E(i, valid ? xs[k][dx] + ry[dy] + pz[dz] : self[k]);
-                raw[k][i] = S::raw_pair(PAIR_BASE(i), e);
+                // at the source: one pair, one base.  (Both candidates are
+                // computed first so that the choice compiles to selects.)
+                const int src = xs[k][dx] + ry[dy] + pz[dz];
+                const uint32_t e_back = PAIR_E(o, self[k]);
+                const uint32_t e_src = PAIR_E(i, valid ? src : self[k]);
+                const uint32_t e_act = redir ? e_back : e_src;
+                raw[k][i] = S::raw_pair(PAIR_BASE(i), act ? e_act : 0u);
             } else {
-                const int idx = !act ? 0 : valid ? xs[k][dx] + ry[dy] + pz[dz] : self[k];
-                raw[k][i] = S::raw(SLOT(i), idx);
+                const int src = xs[k][dx] + ry[dy] + pz[dz];
+                const int idx = valid ? src : self[k];
+                raw[k][i] = S::raw(SLOT(i), act ? idx : 0);
             }
         }
         if constexpr (TH) {
@@ -357,8 +360,10 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 }
                 const int dx = 1 - L::c[q][0], dy = 1 - L::c[q][1], dz = 1 - L::c[q][2];
                 const bool pull = OK(k, dx, dy, dz) && !SOLID(k, o);
-                const uint32_t e = !act ? 0u : pull ? PAIR_E(q, xs[k][dx] + ry[dy] + pz[dz]) : PAIR_E(o, self[k]);
-                graw[q] = a.gslot[q < o ? q : o][e];
+                const uint32_t e_src = PAIR_E(q, xs[k][dx] + ry[dy] + pz[dz]);
+                const uint32_t e_back = PAIR_E(o, self[k]);
+                const uint32_t e_act = pull ? e_src : e_back;
+                graw[q] = a.gslot[q < o ? q : o][act ? e_act : 0u];
             }
         }
     }
@@ -567,15 +572,14 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 //      = (w rho)(cu (3 + 9/2 cu) - 3/2 u^2) + w d
                 const float e = __builtin_fmaf(w * rho, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1), w * d);
                 float v;
-                if (ext) {
-                    v = e;
-                } else {
-                    v = __builtin_fmaf(e - fi[k][i], inv_tau, fi[k][i]);   // h - (h - h_eq)/tau
-                    if (a.source_on) {
-                        const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
-                        v += (pre * w) * (__builtin_fmaf(9.0f * cf, cu, 3.0f * cf) - 3.0f * fu);
-                    }
+                // h - (h - h_eq)/tau + Guo source, or h_eq past the reset
+                // threshold (a select, not a per-direction branch)
+                v = __builtin_fmaf(e - fi[k][i], inv_tau, fi[k][i]);
+                if (a.source_on) {
+                    const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
+                    v += (pre * w) * (__builtin_fmaf(9.0f * cf, cu, 3.0f * cf) - 3.0f * fu);
                 }
+                v = ext ? e : v;
                 if (ODD) {
                     S::store_b(SLOT(i), opaque_offset(self[k], sizeof(T)), v);
                 } else if (i == 0) {
@@ -598,8 +602,11 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                         LDS(i, xs[k][dx]) = ok && !red ? v : __uint_as_float(kNoValue);
                         if (ok && red) S::store(SLOT(i), self[k], vb);
                     } else {
-                        const uint32_t e = red ? PAIR_E(i, self[k]) : PAIR_E(o, xs[k][dx] + ry[dy] + pz[dz]);
-                        if (ok) S::store_pair(PAIR_BASE(i), e, red ? vb : v);
+                        const uint32_t e_back = PAIR_E(i, self[k]);
+                        const uint32_t e_dst = PAIR_E(o, xs[k][dx] + ry[dy] + pz[dz]);
+                        const uint32_t e = red ? e_back : e_dst;
+                        const float out = red ? vb : v;
+                        if (ok) S::store_pair(PAIR_BASE(i), e, out);
                     }
                 } else {
                     // no walls: slot o at x + c_i; writes leaving the domain are
@@ -641,8 +648,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                         const bool ok = OK(k, dx, dy, dz), sol = SOLID(k, q);
                         float vv = v;
                         if (ok && sol) vv = __builtin_fmaf(2.0f * L::wt[q], WALL(k, q).temperature, -v);
-                        const uint32_t e = (!ok || sol) ? PAIR_E(q, self[k]) : PAIR_E(o, xs[k][dx] + ry[dy] + pz[dz]);
-                        a.gslot[q < o ? q : o][e] = vv;
+                        const uint32_t e_here = PAIR_E(q, self[k]);
+                        const uint32_t e_dst = PAIR_E(o, xs[k][dx] + ry[dy] + pz[dz]);
+                        a.gslot[q < o ? q : o][(!ok || sol) ? e_here : e_dst] = vv;
                     }
                 }
             }
@@ -779,7 +787,10 @@ hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const
                           double *partial, hipStream_t stream) {
     constexpr int VS = BF_VS;
     constexpr int CPT = cells_per_thread(ST, false, false);
-    const bool al = !ODD && sh.lds_bytes > 0;
+    // The row buffer pays off for the plain kernels and for FP32 walls; with
+    // walls and 16-bit storage the direct accesses are faster (the barriers
+    // stall blocks whose loads wait on the solid-neighbour mask).
+    const bool al = !ODD && sh.lds_bytes > 0 && !(walls && ST != ST_FP32);
     if (!walls && !mp && !th && a.periodic_x && a.periodic_y && a.periodic_z) {
         if constexpr (!ODD) {
             if (al) {
