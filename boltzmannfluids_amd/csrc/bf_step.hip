@@ -240,6 +240,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     bool vx[CPT][3];
     int self[CPT];
     bool active[CPT];
+    bool in_bounds[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int x = (blockIdx.x * CPT + k) * blockDim.x + tx;
@@ -248,6 +249,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         vx[k][0] = !w0 || a.periodic_x != 0; vx[k][1] = true; vx[k][2] = !w2 || a.periodic_x != 0;
         self[k] = x + ry[1] + pz[1];
         active[k] = x < X && y < Y;
+        in_bounds[k] = active[k];
     }
 
     // Shan-Chen: stage psi of the previous tick for the block tile + halo.
@@ -322,7 +324,10 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         for (int i = 0; i < Q; ++i) {
             const int o = opposite(i);
             if constexpr (ODD) {
-                raw[k][i] = S::raw_b(SLOT(o), opaque_offset(act ? self[k] : 0, sizeof(T)));
+                // every cell of the grid loads its own slots: the addresses do
+                // not wait for the boundary id (a solid cell shares its lines
+                // with the fluid cells of its wave)
+                raw[k][i] = S::raw_b(SLOT(o), opaque_offset(in_bounds[k] ? self[k] : 0, sizeof(T)));
                 continue;
             }
             if (i == 0) {
@@ -354,7 +359,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             for (int q = 0; q < QT; ++q) {
                 const int o = opposite(q);
                 if (ODD || q == 0) {
-                    if constexpr (ODD) graw[q] = *at_bytes(a.gslot[o], opaque_offset(act ? self[k] : 0, 4));
+                    if constexpr (ODD) graw[q] = *at_bytes(a.gslot[o], opaque_offset(in_bounds[k] ? self[k] : 0, 4));
                     else graw[q] = *at(a.gslot[0], act ? self[k] : 0);
                     continue;
                 }
