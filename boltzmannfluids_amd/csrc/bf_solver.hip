@@ -68,6 +68,15 @@ struct bf_solver {
 
 static std::string g_create_error;
 
+// Slot stride: the cell count rounded up to 64 elements (256-byte aligned
+// slots) plus a pad of 16384 elements, which places the slot bases at
+// different offsets modulo the memory interleave (measured 0.6-2 % faster on
+// the 256^3 and 384^3 D3Q19 workloads).
+constexpr int64_t kSlotPad = 16384;
+// Largest grid: two 16-bit slots, (2 N + 63 + kSlotPad) elements, stay below
+// 4 GiB of 32-bit byte offsets.
+constexpr int64_t kMaxCells = (int64_t(1) << 30) - (int64_t(1) << 14);
+
 namespace {
 
 bf_status fail(bf_solver *s, bf_status st, const char *fmt, ...) {
@@ -305,11 +314,12 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
     if (!(cfg->relaxation_time > 0.5f))
         return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: relaxation time %g must exceed 0.5", (double)cfg->relaxation_time);
     const int64_t N = (int64_t)X * Y * Z;
-    // each slot (stride cells of up to 4 bytes) is addressed with 32-bit byte
-    // offsets; the reference's uint32 element addressing (collide.comp:196-200)
-    // allows Q * N < 2^32 only
-    if (N > (int64_t(1) << 30) - 64)
-        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: %lld cells exceed the 2^30 cell limit", (long long)N);
+    // each slot (stride cells of up to 4 bytes) and each pair of opposite
+    // 16-bit slots is addressed with 32-bit byte offsets; the reference's
+    // uint32 element addressing (collide.comp:196-200) allows Q * N < 2^32 only
+    if (N > kMaxCells)
+        return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: %lld cells exceed the limit of %lld",
+                    (long long)N, (long long)kMaxCells);
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
@@ -325,7 +335,7 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
     s->qt = (cfg->velocity_set == BF_D2Q9) ? 5 : 7;
     s->X = X; s->Y = Y; s->Z = Z;
     s->N = N;
-    s->stride = (N + 63) / 64 * 64;
+    s->stride = (N + 63) / 64 * 64 + kSlotPad;
     s->storage = cfg->storage;
     s->elem = cfg->storage == BF_STORE_FP32 ? 4 : 2;
     s->tau = cfg->relaxation_time;

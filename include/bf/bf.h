@@ -149,7 +149,8 @@ bf_status bf_velocity_table(int32_t set, int32_t thermal, float *out, int32_t ca
 /* Create a solver: validates the config and allocates the lattice.
  * Replaces the allocation half of LBM::initialize_fields
  * (LBM.cpp:415-462 and _generate_lattice_buffer, LBM.cpp:808-876).
- * At most 2^30 - 64 cells (each population slot stays below 4 GiB). */
+ * At most 2^30 - 2^14 cells (each pair of opposite 16-bit population slots,
+ * and each FP32 slot, stays below 4 GiB). */
 bf_status bf_create(const bf_config *cfg, bf_solver **out);
 
 /* Release everything the handle owns.  Safe on NULL. */
