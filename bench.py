@@ -40,10 +40,11 @@ def bytes_per_cell(q, storage, extra=0):
     return 2 * q * BYTES[storage] + extra
 
 
-def load_traffic(storage, size):
+def load_traffic(name):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
-    the copy-kernel calibration) if a summary for this workload is committed."""
-    path = os.path.join(ROOT, "profiles", f"step_d3q19_{storage}_{size}_traffic.json")
+    the copy-kernel calibration) if a summary for this workload is committed
+    (profiles/<name>_traffic.json)."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -84,7 +85,8 @@ def run_tgv(bfa, n, storage, steps, warmup):
     cells = n ** 3
     kernel_ms = ev / steps
     return {"mlups": cells * steps / wall / 1e6, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kernel_ms,
-            "roofline": roofline(bytes_per_cell(19, storage), cells, kernel_ms, load_traffic(storage, n))}
+            "roofline": roofline(bytes_per_cell(19, storage), cells, kernel_ms,
+                                 load_traffic(f"step_d3q19_{storage}_{n}"))}
 
 
 def field_error(a, b):
@@ -151,7 +153,8 @@ def porous_config(bfa, n=384, steps=100, warmup=10, err_n=128, err_ticks=1000):
         kms = ev / steps
         out["storage"][st] = {"mlups": round(cells * steps / wall / 1e6, 1),
                               "fluid_mlups": round(fluid * steps / wall / 1e6, 1),
-                              "roofline": roofline(bytes_per_cell(19, st, 1), cells, kms)}
+                              "roofline": roofline(bytes_per_cell(19, st, 1), cells, kms,
+                                                   load_traffic(f"step_d3q19_porous_{st}_{n}"))}
         if st != "fp32":
             er, eu = field_error(run_fields(bfa, shape, 0.56, st, err_ticks, setup)[mask], ref)
             out["storage"][st]["error_vs_fp32"] = {"rho": er, "u": eu, "workload": f"{err_n}^3, {err_ticks} ticks"}
@@ -186,7 +189,8 @@ def spinodal_thermal_config(bfa, n=192, steps=100, warmup=10):
         # the synchronous pre-pass reads the lattices once more (27 + 7 slots)
         extra = (27 + 7) * 4 + 4 if mode == "synchronous" else 0
         out[mode] = {"mlups": round(cells * steps / wall / 1e6, 1),
-                     "roofline": roofline(2 * 27 * 4 + 2 * 7 * 4 + 8 + 1 + extra, cells, kms),
+                     "roofline": roofline(2 * 27 * 4 + 2 * 7 * 4 + 8 + 1 + extra, cells, kms,
+                                          load_traffic(f"step_d3q27_shanchen_thermal_{mode}_{n}")),
                      "fp64_mass_relative_drift": abs(m1 - m0) / m0}
     return out
 
@@ -305,6 +309,25 @@ def porous_sample(bfa, ticks, n=384):
     print(f"porous sample: {n}^3, {ticks} ticks per storage type", flush=True)
 
 
+def spinodal_sample(bfa, ticks, n=192):
+    """Short workload for rocprofv3: `ticks` ticks of the D3Q27 Shan-Chen +
+    D3Q7 thermal workload with the lagged pseudopotential."""
+    import torch  # noqa: F401  (one HIP runtime with torch)
+    rho, u, ids = spinodal(n, n, n)
+    cells = n ** 3
+    s = bfa.Solver((n, n, n), 1.0, periodic=(1, 0, 1), velocity_set="D3Q27", multiphase=True)
+    s.set_object(1, temperature=1.0)
+    s.set_object(2, temperature=0.0)
+    s.set_param("interaction_strength", -5.0)
+    s.set_param("reference_temperature", 0.5)
+    s.init_fields(rho, u, force=np.tile(np.array([0, -1e-5, 0], np.float32), (cells, 1)),
+                  temperature=np.full(cells, 0.5, np.float32), boundary_id=ids)
+    s.step(ticks)
+    s.synchronize()
+    s.close()
+    print(f"spinodal sample: {n}^3, {ticks} ticks", flush=True)
+
+
 def adjoint_sample(bfa, ticks):
     """Short workload for rocprofv3: the D3Q19 128^3 channel forward and its
     gradient over `ticks` ticks with checkpoints only (K = 100)."""
@@ -334,6 +357,8 @@ def main():
                     help="run only the short workload used for rocprofv3 --pmc passes")
     ap.add_argument("--porous-sample", type=int, default=0, metavar="TICKS",
                     help="run only TICKS ticks of the 384^3 porous duct per storage type (for rocprofv3)")
+    ap.add_argument("--spinodal-sample", type=int, default=0, metavar="TICKS",
+                    help="run only TICKS ticks of the 192^3 Shan-Chen + thermal workload (for rocprofv3)")
     ap.add_argument("--adjoint-sample", type=int, default=0, metavar="TICKS",
                     help="run only the D3Q19 128^3 channel gradient over TICKS ticks (for rocprofv3)")
     args = ap.parse_args()
@@ -348,6 +373,9 @@ def main():
         return
     if args.porous_sample:
         porous_sample(bfa, args.porous_sample)
+        return
+    if args.spinodal_sample:
+        spinodal_sample(bfa, args.spinodal_sample)
         return
     if args.adjoint_sample:
         adjoint_sample(bfa, args.adjoint_sample)
