@@ -187,6 +187,12 @@ constexpr uint32_t kNoValue = 0x7fa5a5a5u;
 #define BF_OCCUPANCY
 #endif
 
+// v modulo n in [0, n).  Tile coordinates are inside the grid except in the
+// halo at the grid's edges, so the integer division only runs on those lanes.
+__device__ inline int wrap_index(int v, int n) {
+    return (unsigned)v < (unsigned)n ? v : ((v % n) + n) % n;
+}
+
 template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER>
 __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
     // PER: every axis periodic (and no walls): every neighbour exists, so the
@@ -259,10 +265,10 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         const int nplanes = D3 ? 3 : 1;
         const int x0 = blockIdx.x * kMpBX - 1, y0 = blockIdx.y * kMpBY - 1;
         for (int k = tid; k < kTileX * kTileY * nplanes; k += kMpBX * kMpBY) {
-            int lx = k % kTileX, ly = (k / kTileX) % kTileY, lz = k / (kTileX * kTileY);
-            int gx = ((x0 + lx) % X + X) % X;
-            int gy = ((y0 + ly) % Y + Y) % Y;
-            int gz = D3 ? ((z - 1 + lz) % Z + Z) % Z : z;
+            const int lx = k % kTileX, ly = (k / kTileX) % kTileY, lz = k / (kTileX * kTileY);
+            const int gx = wrap_index(x0 + lx, X);
+            const int gy = wrap_index(y0 + ly, Y);
+            const int gz = D3 ? wrap_index(z - 1 + lz, Z) : z;
             tile[k] = a.psi_prev[gx + gy * X + gz * XY];
         }
         __syncthreads();
