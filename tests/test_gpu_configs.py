@@ -1,0 +1,169 @@
+"""The benchmark workloads at their full size and tick counts, on the GPU:
+the properties each one is specified by (analytic decay, conservation, the
+16-bit fields against the FP32 run, the phase behaviour, the adjoint against
+finite differences).  The FP64 oracle cannot run these sizes; the runs at
+oracle size are in test_gpu_long.py and test_gpu_parity.py."""
+import sys
+
+import numpy as np
+import pytest
+
+from boltzmannfluids_amd.scenarios import porosity, porous_spheres, spinodal, taylor_green
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def kinetic_energy(vd):
+    v = vd.astype(np.float64)
+    return 0.5 * np.sum(v[:, 3] * (v[:, :3] ** 2).sum(1))
+
+
+def test_tgv_256_1000_ticks(bfa):
+    """D3Q19 256^3 periodic Taylor-Green vortex, tau 0.55, u0 0.05, 1000 ticks.
+    FP32: kinetic energy every 100 ticks against E0 exp(-6 nu k^2 t) (the
+    Stokes decay; at Re = u0 / (k nu) = 122 the nonlinear transfer stays small
+    over 1.2 turnover times) and the FP64 mass sum.  FP16 and BF16 storage:
+    the fields after 1000 ticks against the FP32 run."""
+    n, ticks, tau = 256, 1000, 0.55
+    rho, u = taylor_green(n, 0.05)
+    nu = (np.float32(tau) - 0.5) / 3
+    k = 2 * np.pi / n
+    fields = {}
+    for st in ("fp32", "fp16", "bf16"):
+        say(f"case: TGV {n}^3 {st} {ticks} ticks")
+        s = bfa.Solver((n, n, n), tau, velocity_set="D3Q19", storage=st)
+        s.init_fields(rho, u)
+        if st == "fp32":
+            m0 = s.conservation()[0]
+            s.step(1, save=True)
+            e0 = kinetic_energy(s.read("velocity_density"))
+            worst = 0.0
+            for t in range(100, ticks + 1, 100):
+                s.step(100 if t > 100 else 99, save=True)
+                e = kinetic_energy(s.read("velocity_density"))
+                worst = max(worst, abs(e / (e0 * np.exp(-6 * nu * k * k * (t - 1))) - 1.0))
+            m1 = s.conservation()[0]
+            say(f"  KE vs Stokes decay: worst relative deviation {worst:.3e}; mass drift {abs(m1 - m0) / m0:.3e}")
+            assert worst < 0.01
+            assert abs(m1 - m0) / m0 < 1e-7
+        else:
+            s.step(ticks, save=True)
+        fields[st] = s.read("velocity_density").astype(np.float64)
+        s.close()
+    ref = fields["fp32"]
+    for st, bar in (("fp16", 1e-3), ("bf16", 1e-2)):
+        er, eu = rel_l2(fields[st][:, 3], ref[:, 3]), rel_l2(fields[st][:, :3], ref[:, :3])
+        say(f"  {st} vs FP32: rho {er:.3e} u {eu:.3e}")
+        assert er < bar and eu < bar
+
+
+def test_porous_duct_384_5000_ticks(bfa):
+    """D3Q19 384^3 porous duct (periodic x, walls y/z, 5 % spheres r 4-12),
+    g_x 1e-6, tau 0.56, 5000 ticks in FP32, FP16 and BF16 storage: the FP64
+    mass sum, the flow along +x, and the 16-bit fields against the FP32 run on
+    the fluid cells.  FP16 stays within 1e-3 of FP32; BF16 does not: with
+    |u| ~ 3e-3 its 8-bit mantissa (a quantum of 2^-8 of h = f - w) biases the
+    mean flow by a few percent, which the bar here states."""
+    n, ticks = 384, 5000
+    ids = porous_spheres(n, n, n)
+    cells = n ** 3
+    fluid = ids == 0
+    fields = {}
+    for st in ("fp32", "fp16", "bf16"):
+        say(f"case: porous duct {n}^3 {st} {ticks} ticks")
+        s = bfa.Solver((n, n, n), 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", storage=st)
+        s.init_fields(np.ones(cells, np.float32), np.zeros((cells, 3), np.float32), boundary_id=ids)
+        s.set_param("constant_force", np.array([1e-6, 0, 0], np.float32))
+        m0 = s.conservation()[0]
+        s.step(ticks, save=True)
+        m1 = s.conservation()[0]
+        vd = s.read("velocity_density")[fluid].astype(np.float64)
+        s.close()
+        say(f"  mass drift {abs(m1 - m0) / m0:.3e}; mean u_x {vd[:, 0].mean():.4e}")
+        assert abs(m1 - m0) / m0 < {"fp32": 1e-6, "fp16": 1e-3, "bf16": 1e-2}[st]
+        assert vd[:, 0].mean() > 0
+        fields[st] = vd
+    ref = fields["fp32"]
+    for st, bar in (("fp16", 1e-3), ("bf16", 6e-2)):
+        er, eu = rel_l2(fields[st][:, 3], ref[:, 3]), rel_l2(fields[st][:, :3], ref[:, :3])
+        say(f"  {st} vs FP32: rho {er:.3e} u {eu:.3e}")
+        assert er < bar and eu < bar
+
+
+def test_shan_chen_thermal_192_20000_ticks(bfa):
+    """D3Q27 Shan-Chen 192^3 (G -5, psi = 1 - exp(-rho), rho = 2 + U(+-0.01),
+    tau 1) with the D3Q7 thermal lattice between a hot (y = 0) and a cold
+    (y = 191) wall and Boussinesq buoyancy, 20000 ticks with the FP64 mass sum
+    taken every 100 ticks.
+    - Mass: the FP64 sum of the FP32 populations drifts by FP32 rounding in
+      the dense phase (h = f - w is not small at rho = 2; the FP32 oracle
+      drifts alike at 32^3), below 2e-4 after 20000 ticks.
+    - Phases: rho = 2 lies above the liquid coexistence density (1.93 at
+      G = -5 with Guo forcing) and outside the spinodal interval (0.32, 1.29)
+      where dp/drho < 0, so the mixture stays one liquid phase: the initial
+      noise decays in the bulk, and the walls (psi of rho_eff = 1) hold a
+      depleted layer.
+    - Temperature stays between the wall values."""
+    n, ticks = 192, 20000
+    rho, u, ids = spinodal(n, n, n)
+    cells = n ** 3
+    say(f"case: Shan-Chen + thermal {n}^3 {ticks} ticks")
+    s = bfa.Solver((n, n, n), 1.0, periodic=(1, 0, 1), velocity_set="D3Q27", multiphase=True)
+    s.set_object(1, temperature=1.0)
+    s.set_object(2, temperature=0.0)
+    s.set_param("interaction_strength", -5.0)
+    s.set_param("reference_temperature", 0.5)
+    s.init_fields(rho, u, force=np.tile(np.array([0, -1e-5, 0], np.float32), (cells, 1)),
+                  temperature=np.full(cells, 0.5, np.float32), boundary_id=ids)
+    m0 = s.conservation()[0]
+    drift = 0.0
+    for t in range(0, ticks, 100):
+        s.step(100, save=(t + 100 == ticks), conservation=True)
+        drift = max(drift, abs(s.last_step_sums()[0] - m0) / m0)
+    fluid = ids == 0
+    r = s.read("velocity_density")[:, 3].astype(np.float64).reshape(n, n, n)
+    T = s.read("force_temperature")[fluid, 3].astype(np.float64)
+    s.close()
+    bulk = r[:, 16:n - 16, :]
+    wall = r[:, 1, :]
+    noise0 = 0.01 / np.sqrt(3.0)
+    say(f"  max mass drift {drift:.3e}; bulk rho {bulk.mean():.4f} +- {bulk.std():.2e} (initial noise {noise0:.2e}); "
+        f"next to the hot wall {wall.mean():.3f}; T in [{T.min():.3f}, {T.max():.3f}]")
+    assert drift < 2e-4
+    assert abs(bulk.mean() - 2.0) < 0.05 and bulk.std() < noise0
+    assert wall.mean() < bulk.mean()
+    assert T.min() > -1e-3 and T.max() < 1.0 + 1e-3
+
+
+def test_channel_d3q19_128_gradient_vs_finite_difference(bfa):
+    """Differentiable D3Q19 128^3 channel, tau 0.6, u_in 0.05, 2000 ticks,
+    checkpoints every 100 (none resident): dJ/du_in of the adjoint against
+    central differences of forward runs, with the O(eps^2) truncation removed
+    by Richardson extrapolation over eps and 2 eps."""
+    res, steps, u_in, eps = (128, 128, 128), 2000, 0.05, 1e-3
+    say(f"case: channel D3Q19 {res} {steps} ticks, adjoint vs central difference")
+    ch = bfa.capi.Channel(res, 0.6, u_in, velocity_set="D3Q19")
+    ch.set_porosity(porosity(*res))
+    J, dg, du = ch.gradient(steps, checkpoint_every=100, trajectory_budget_bytes=0)
+
+    def central(e):
+        ch.set_inlet_velocity(u_in + e)
+        jp = ch.forward(steps)
+        ch.set_inlet_velocity(u_in - e)
+        jm = ch.forward(steps)
+        return (jp - jm) / (2 * e)
+
+    fd1, fd2 = central(eps), central(2 * eps)
+    ch.close()
+    fd = (4 * fd1 - fd2) / 3
+    say(f"  J {J:.6e}  dJ/du_in adjoint {du:.6e}  central {fd1:.6e} (eps) {fd2:.6e} (2 eps)  "
+        f"extrapolated {fd:.6e}  rel {abs(du - fd) / abs(fd):.2e}")
+    assert abs(du - fd) / abs(fd) < 1e-4
+    assert np.isfinite(dg).all() and np.abs(dg).max() > 0
