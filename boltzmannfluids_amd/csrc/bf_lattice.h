@@ -6,6 +6,15 @@
 // in-place update relies on (collide.comp:219-222).  The thermal sets are the
 // leading 5 (2-D) / 7 (3-D) directions of the momentum sets, so a thermal
 // direction k points the same way as momentum direction k.
+//
+// rest_fix / rest_fix_t: 1 minus the exact sum of the FP32 weights (a power of
+// two, exact in FP32).  The floats nearest to the rational weights sum to a
+// little more than 1, and the equilibrium's mass part sum_i w_i (rho - 1)
+// would gain (rho - 1)(sum - 1) per cell and tick (7e-5 of the mass after
+// 20000 ticks of a rho = 2 liquid).  The rest population's mass part is
+// computed as fma(w_0, rho - 1, rest_fix (rho - 1)), one rounding of
+// (w_0 + rest_fix)(rho - 1), which leaves the moving weights, and with them
+// the lattice's isotropy, untouched.
 #pragma once
 
 #include <stdint.h>
@@ -29,6 +38,7 @@ template <> struct Lattice<D2Q9> {
     static constexpr float w[9] = {float(4.0 / 9), float(1.0 / 9), float(1.0 / 9), float(1.0 / 9),
                                    float(1.0 / 9), float(1.0 / 36), float(1.0 / 36), float(1.0 / 36),
                                    float(1.0 / 36)};
+    static constexpr float rest_fix = -0x1p-27f, rest_fix_t = -0x1p-25f;
     static constexpr float wt[5] = {float(1.0 / 3), float(1.0 / 6), float(1.0 / 6), float(1.0 / 6),
                                     float(1.0 / 6)};
 };
@@ -42,6 +52,7 @@ template <> struct Lattice<D3Q15> {
                                     float(1.0 / 9),  float(1.0 / 9),  float(1.0 / 9),  float(1.0 / 72),
                                     float(1.0 / 72), float(1.0 / 72), float(1.0 / 72), float(1.0 / 72),
                                     float(1.0 / 72), float(1.0 / 72), float(1.0 / 72)};
+    static constexpr float rest_fix = -0x1p-27f, rest_fix_t = 0.0f;
     static constexpr float wt[7] = {float(1.0 / 4), float(1.0 / 8), float(1.0 / 8), float(1.0 / 8),
                                     float(1.0 / 8), float(1.0 / 8), float(1.0 / 8)};
 };
@@ -57,6 +68,7 @@ template <> struct Lattice<D3Q19> {
                                     float(1.0 / 36), float(1.0 / 36), float(1.0 / 36), float(1.0 / 36),
                                     float(1.0 / 36), float(1.0 / 36), float(1.0 / 36), float(1.0 / 36),
                                     float(1.0 / 36), float(1.0 / 36), float(1.0 / 36)};
+    static constexpr float rest_fix = -0x1p-26f, rest_fix_t = 0.0f;
     static constexpr float wt[7] = {float(1.0 / 4), float(1.0 / 8), float(1.0 / 8), float(1.0 / 8),
                                     float(1.0 / 8), float(1.0 / 8), float(1.0 / 8)};
 };
@@ -75,6 +87,7 @@ template <> struct Lattice<D3Q27> {
         float(1.0 / 54),  float(1.0 / 54),  float(1.0 / 54),  float(1.0 / 54),  float(1.0 / 216),
         float(1.0 / 216), float(1.0 / 216), float(1.0 / 216), float(1.0 / 216), float(1.0 / 216),
         float(1.0 / 216), float(1.0 / 216)};
+    static constexpr float rest_fix = -0x1p-27f, rest_fix_t = 0.0f;
     static constexpr float wt[7] = {float(1.0 / 4), float(1.0 / 8), float(1.0 / 8), float(1.0 / 8),
                                     float(1.0 / 8), float(1.0 / 8), float(1.0 / 8)};
 };

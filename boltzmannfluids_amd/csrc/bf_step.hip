@@ -580,8 +580,11 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 const int o = opposite(i);
                 const float cu = (i == 0) ? 0.0f : dotc<BF_VS>(i, ux, uy, uz);
                 // h_eq = w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2) - w
-                //      = (w rho)(cu (3 + 9/2 cu) - 3/2 u^2) + w d
-                const float e = __builtin_fmaf(w * rho, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1), w * d);
+                //      = (w rho)(cu (3 + 9/2 cu) - 3/2 u^2) + w d,
+                // the rest population's mass part with w_0 + rest_fix, rounded
+                // once (rest_fix * d is exact, w_0 d is not rounded before the sum)
+                const float md = (i == 0) ? __builtin_fmaf(w, d, L::rest_fix * d) : w * d;
+                const float e = __builtin_fmaf(w * rho, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1), md);
                 float v;
                 // h - (h - h_eq)/tau + Guo source, or h_eq past the reset
                 // threshold (a select, not a per-direction branch)
@@ -646,7 +649,8 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                     const int o = opposite(q);
                     const float cu = (q == 0) ? 0.0f : dotc<BF_VS>(q, ux, uy, uz);
                     // wt T (1 + cu/cs2 + cu^2/(2 cs2^2) - u^2/(2 cs2))
-                    const float e = (L::wt[q] * temp) * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
+                    const float wT = (q == 0) ? __builtin_fmaf(L::wt[0], temp, L::rest_fix_t * temp) : L::wt[q] * temp;
+                    const float e = wT * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
                     const float v = __builtin_fmaf(e - gi[q], inv_tau_t, gi[q]);
                     if (ODD) {
                         *at_bytes(a.gslot[q], opaque_offset(self[k], 4)) = v;
@@ -750,7 +754,8 @@ __global__ void __launch_bounds__(256) k_init(const InitArgs a) {
     for (int i = 0; i < L::Q; ++i) {
         const float cu = i == 0 ? 0.0f : dotc<VS>(i, ux, uy, uz);
         const float w = L::w[i];
-        float e = __builtin_fmaf(w * r, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1), w * d);
+        const float md = (i == 0) ? __builtin_fmaf(w, d, L::rest_fix * d) : w * d;
+        float e = __builtin_fmaf(w * r, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), c1), md);
         // round to FP32 before the storage conversion: without the empty asm the
         // FMA and an FP16 conversion fuse into v_fma_mixlo_f16, which rounds once
         if constexpr (ST == ST_FP16) asm("" : "+v"(e));
@@ -763,7 +768,8 @@ __global__ void __launch_bounds__(256) k_init(const InitArgs a) {
 #pragma unroll
         for (int k = 0; k < L::QT; ++k) {
             const float cu = k == 0 ? 0.0f : dotc<VS>(k, ux, uy, uz);
-            a.g[(size_t)k * a.stride + c] = (L::wt[k] * T0) * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
+            const float wT = (k == 0) ? __builtin_fmaf(L::wt[0], T0, L::rest_fix_t * T0) : L::wt[k] * T0;
+            a.g[(size_t)k * a.stride + c] = wT * __builtin_fmaf(cu, __builtin_fmaf(cu, k2, k1), t1);
         }
     }
     if (a.psi0) {

@@ -127,6 +127,19 @@ static const dir_t *table_of(int set, int thermal, int *q) {
     *q = 0; return NULL;
 }
 
+/* 1 minus the exact sum of the FP32 weights of a set.  The floats nearest to
+ * the rational weights sum to a little more than 1 (by 2^-27 ... 2^-25), so
+ * the equilibrium's mass part sum_i w_i (rho - 1) would gain (rho - 1)(sum - 1)
+ * per cell and tick; the FP32 build, like the HIP kernel, computes the rest
+ * population's mass part with w_0 + this correction.  (The sum of at most 27
+ * floats and 1 minus it are exact in double; the result is a power of two.) */
+static float rest_fix_f32(int set, int thermal) {
+    int q; const dir_t *t = table_of(set, thermal, &q);
+    double sum = 0;
+    for (int i = 0; i < q; ++i) sum += (double)(float)t[i].w;
+    return (float)(1.0 - sum);
+}
+
 /* collide.comp:219-222: opposite of i is i+1 for odd i, i-1 for even i, 0 for 0 */
 static int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
 
@@ -186,6 +199,7 @@ typedef struct orc_solver {
     long N;
     int q, qt;
     const dir_t *T, *TT;
+    REAL fix, fix_t;   /* rest-population mass correction (rest_fix), 0 in the FP64 build */
     int vset, storage, multiphase, bb_mode;
     int sc_sync;       /* synchronous Shan-Chen pseudopotential (density pre-pass) */
     int per[3];
@@ -255,15 +269,15 @@ static int is_solid(const orc_solver *s, long c) { return s->has_ids && s->ids[c
  * with exact 3, 9/2, 3/2 as in set_equilibrium_populations.comp:179-196):
  *   w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2) - w = (w rho) (cu (3 + 9/2 cu) + c1) + w d,
  * with c1 = -3/2 u^2 computed once per cell and d = rho - 1. */
-static REAL heq(REAL w, REAL rho, REAL d, REAL cu, REAL c1) {
-    return FMA(w * rho, FMA(cu, FMA((REAL)4.5, cu, (REAL)3), c1), w * d);
+static REAL heq(REAL w, REAL rho, REAL d, REAL cu, REAL c1, REAL fix) {
+    return FMA(w * rho, FMA(cu, FMA((REAL)4.5, cu, (REAL)3), c1), FMA(w, d, fix * d));
 }
 
 /* Thermal equilibrium with a configurable c_s^2 (collide.comp:634-651):
  *   w T (1 + cu/cs2 + cu^2/(2 cs2^2) - u^2/(2 cs2)) = (w T) (t1 + cu (k1 + cu k2)),
  * k1 = 1/cs2, k2 = k1^2/2, t1 = 1 - u^2 k1/2. */
-static REAL geq(REAL w, REAL T, REAL cu, REAL k1, REAL k2, REAL t1) {
-    return (w * T) * FMA(cu, FMA(cu, k2, k1), t1);
+static REAL geq(REAL w, REAL T, REAL cu, REAL k1, REAL k2, REAL t1, REAL fix) {
+    return FMA(w, T, fix * T) * FMA(cu, FMA(cu, k2, k1), t1);
 }
 
 /* u.u = ((ux ux) + uy uy) + uz uz with FMAs */
@@ -426,7 +440,7 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
                 const int *c = s->T[i].c;
                 REAL w = (REAL)s->T[i].w;
                 REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
-                REAL e = heq(w, rho, d, cu, c1);
+                REAL e = heq(w, rho, d, cu, c1, i == 0 ? s->fix : (REAL)0);
                 REAL v;
                 if (ext) v = e;
                 else {
@@ -449,7 +463,7 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
             for (int k = 0; k < s->qt; ++k) {
                 const int *c = s->TT[k].c;
                 REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
-                REAL e = geq((REAL)s->TT[k].w, T, cu, k1, k2, t1);
+                REAL e = geq((REAL)s->TT[k].w, T, cu, k1, k2, t1, k == 0 ? s->fix_t : (REAL)0);
                 REAL v = FMA(e - g[k], inv_tau_t, g[k]);
                 if (update) write_th(s, x, y, z, k, odd, v);
             }
@@ -478,6 +492,8 @@ ORC_API int orc_velocity_table(int set, int thermal, double *out) {
     return q;
 }
 
+ORC_API float orc_rest_fix_f32(int set, int thermal) { return rest_fix_f32(set, thermal); }
+
 ORC_API int orc_opposite(int i) { return opp(i); }
 
 /* Threads the parallel loops use (1 without OpenMP). */
@@ -500,6 +516,12 @@ ORC_API orc_solver *orc_create(int X, int Y, int Z, int vset, int storage, int m
     s->multiphase = multiphase; s->bb_mode = bb_mode;
     s->per[0] = px; s->per[1] = py; s->per[2] = pz;
     s->TT = table_of(vset == V_D2Q9 ? V_D2Q5 : V_D3Q7, 1, &s->qt);
+#ifdef ORACLE_F32
+    s->fix = (REAL)rest_fix_f32(vset, 0);
+    s->fix_t = (REAL)rest_fix_f32(vset == V_D2Q9 ? V_D2Q5 : V_D3Q7, 1);
+#else
+    s->fix = 0; s->fix_t = 0;
+#endif
     s->tau = (REAL)tau; s->G = (REAL)-6.0; s->beta = (REAL)0.5; s->tau_t = (REAL)2;
     s->t_ref = (REAL)1; s->cs2_t = (REAL)(1.0 / 3.0); s->ulim = (REAL)0.25; s->uext = (REAL)0.30;
     for (int i = 0; i < 256; ++i) { s->obj[i][3] = 1; s->obj[i][4] = 1; }
@@ -597,7 +619,7 @@ ORC_API int orc_init_fields(orc_solver *s, const double *rho, const double *u, c
         for (int i = 0; i < s->q; ++i) {
             const int *cc = s->T[i].c;
             REAL cu = (REAL)cc[0] * uu3[0] + (REAL)cc[1] * uu3[1] + (REAL)cc[2] * uu3[2];
-            F_AT(s, i, c) = store_val(s, i, heq((REAL)s->T[i].w, r, r - (REAL)1, cu, c1));
+            F_AT(s, i, c) = store_val(s, i, heq((REAL)s->T[i].w, r, r - (REAL)1, cu, c1, i == 0 ? s->fix : (REAL)0));
         }
     }
     free(s->g); s->g = NULL;
@@ -613,7 +635,7 @@ ORC_API int orc_init_fields(orc_solver *s, const double *rho, const double *u, c
             for (int k = 0; k < s->qt; ++k) {
                 const int *cc = s->TT[k].c;
                 REAL cu = (REAL)cc[0] * uu3[0] + (REAL)cc[1] * uu3[1] + (REAL)cc[2] * uu3[2];
-                G_AT(s, k, c) = geq((REAL)s->TT[k].w, T0, cu, k1, k2, t1);
+                G_AT(s, k, c) = geq((REAL)s->TT[k].w, T0, cu, k1, k2, t1, k == 0 ? s->fix_t : (REAL)0);
             }
         }
     }

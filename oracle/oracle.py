@@ -37,6 +37,7 @@ def load(precision="f64"):
     vp, dp, i32, i64 = C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_long
     sig = {
         "orc_velocity_table": (i32, [i32, i32, dp]),
+        "orc_rest_fix_f32": (C.c_float, [i32, i32]),
         "orc_opposite": (i32, [i32]),
         "orc_threads": (i32, []),
         "orc_create": (vp, [i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, C.c_double]),

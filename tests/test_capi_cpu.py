@@ -3,6 +3,7 @@ input before touching a device.  CPU only: no compute calls."""
 import ctypes as C
 import os
 import re
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -50,6 +51,23 @@ def test_product_thermal_tables_equal_oracle(bfa, orc, name):
     b = orc.velocity_table(name, thermal=True)
     np.testing.assert_array_equal(a[:, :3], b[:, :3])
     np.testing.assert_array_equal(a[:, 3], b[:, 3].astype(np.float32))
+
+
+@pytest.mark.parametrize("name,thermal", [("D2Q9", False), ("D3Q15", False), ("D3Q19", False), ("D3Q27", False),
+                                          ("D2Q5", True), ("D3Q7", True)])
+def test_rest_population_mass_fix(orc, name, thermal):
+    """The FP32 weights plus the rest-population correction sum to exactly 1;
+    the correction is a power of two (zero for D3Q7), and without it the FP32
+    weights do not sum to 1 (the case the correction exists for)."""
+    w = orc.velocity_table(name, thermal=thermal)[:, 3].astype(np.float32)
+    vid = {"D2Q5": 0, "D3Q7": 1}[name] if thermal else orc.VELOCITY_SETS[name]
+    fix = np.float32(orc.load("f32").orc_rest_fix_f32(vid, 1 if thermal else 0))
+    total = sum(Fraction(float(v)) for v in w)
+    assert total + Fraction(float(fix)) == 1
+    if name == "D3Q7":
+        assert fix == 0
+    else:
+        assert total != 1 and fix < 0 and np.log2(-float(fix)) == int(np.log2(-float(fix)))
 
 
 def test_bits_per_boundary_matches_oracle(bfa, orc):

@@ -105,8 +105,8 @@ def test_shan_chen_thermal_2000_ticks(bfa, orc):
         runs.append((s, m0, s.conservation()[0]))
     (g, gm0, gm1), (o, om0, om1) = runs
     say(f"  mass drift GPU {abs(gm1 - gm0) / gm0:.3e} oracle {abs(om1 - om0) / om0:.3e}")
-    # the liquid holds rho ~ 2 (h is not small there), so FP32 rounding drifts
-    # the mass by a few 1e-6 over 2000 ticks, on the GPU as in the FP32 oracle
+    # FP32 rounding in the rho ~ 2 liquid (h is not small there) leaves a few
+    # 1e-6 over 2000 ticks, on the GPU as in the FP32 oracle
     assert abs(gm1 - gm0) / gm0 < 1e-5
     a, b = g.read("velocity_density").astype(np.float64), o.read("velocity_density")
     ta, tb = g.read("force_temperature")[:, 3].astype(np.float64), o.read("force_temperature")[:, 3]
