@@ -21,11 +21,27 @@ struct alignas(16) ObjectEntry {
 
 constexpr int kMaxQ = 27, kMaxQT = 7;
 
+// Lattice layout.  Slot i of cell c is element pop_index(c) of the array that
+// starts at slot[i], with pop_index(c) = c + (c >> slab_log) * slab_skip:
+//  - slabs: the cells are cut into slabs of 2^slab_log cells and the slots
+//    into groups of W adjacent slots (opposite pairs never split); a group
+//    stores, slab after slab, its W slots of that slab one after the other,
+//    so slab_skip = (W - 1) 2^slab_log, and slot i + 1 sits 2^slab_log
+//    elements after slot i within a group (pair_step);
+//  - plain structure of arrays: slab_log = 31 (no cell reaches the second
+//    slab), slab_skip = 0, pair_step = the slot stride.
+// Every group stays below 4 GiB, so byte offsets from a slot base are 32-bit.
+#ifndef BF_SLAB_LOG
+#define BF_SLAB_LOG 13
+#endif
+constexpr int kSlabLog = BF_SLAB_LOG;  // 2^13 cells per slab
+
 struct StepArgs {
-    void *f;                 // lattice, Q slots of `stride` elements (storage type)
+    void *f;                 // lattice (storage type)
     float *g;                // thermal lattice, QT slots of `stride` floats (or null)
-    int64_t stride;          // elements between slots
-    void *slot[kMaxQ];       // base of slot i (f + i * stride)
+    int64_t stride;          // elements between thermal slots
+    void *slot[kMaxQ];       // base of slot i
+    uint32_t slab_log, slab_skip, pair_step;
     float *gslot[kMaxQT];    // base of thermal slot q (g + q * stride)
     int X, Y, Z;
     int periodic_x, periodic_y, periodic_z;
@@ -55,9 +71,10 @@ struct StepArgs {
 };
 
 struct InitArgs {
-    void *f;
+    void *slot[kMaxQ];       // as in StepArgs
+    uint32_t slab_log, slab_skip;
     float *g;
-    int64_t stride;
+    int64_t stride;          // elements between thermal slots
     int64_t n;
     const float *rho;
     const float *u;          // 3 per cell

@@ -7,12 +7,13 @@
 //
 // bf_probe_inplace_bandwidth reproduces the memory traffic of one lattice tick
 // without its arithmetic: every thread reads one 4-byte element of each of
-// `slots` arrays of `cells` elements (structure-of-arrays, slot stride rounded
-// to 64 cells like the solver's lattice) and writes a value back to the same
-// places.  Its rate is the ceiling of the in-place tick's access pattern.
+// `slots` slots of its cell, in the solver's slab layout (slabs of 2^kSlabLog
+// cells holding the slots one after the other), and writes a value back to the
+// same places.  Its rate is the ceiling of the in-place tick's access pattern.
 #include <hip/hip_runtime.h>
 
 #include "../../include/bf/bf.h"
+#include "bf_kernels.h"
 
 namespace {
 
@@ -28,29 +29,31 @@ constexpr int kMaxSlots = 27;
 
 // all loads of a thread are issued before its stores, as in the tick kernel
 template <int NS>
-__global__ void __launch_bounds__(256) k_inplace(float *__restrict__ f, int64_t stride, int64_t cells) {
+__global__ void __launch_bounds__(256) k_inplace(float *__restrict__ f, int64_t cells) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= cells) return;
+    constexpr int64_t P = int64_t(1) << bf::kSlabLog;
+    float *p = f + (c >> bf::kSlabLog) * (NS * P) + (c & (P - 1));
     float v[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) v[s] = __builtin_nontemporal_load(f + s * stride + c);
+    for (int s = 0; s < NS; ++s) v[s] = __builtin_nontemporal_load(p + s * P);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) __builtin_nontemporal_store(0.5f * v[s] + v[(s + 1) % NS], f + s * stride + c);
+    for (int s = 0; s < NS; ++s) __builtin_nontemporal_store(0.5f * v[s] + v[(s + 1) % NS], p + s * P);
 }
 
 template <int NS>
-void launch_inplace(float *f, int64_t stride, int64_t cells, hipStream_t s) {
-    hipLaunchKernelGGL((k_inplace<NS>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, f, stride, cells);
+void launch_inplace(float *f, int64_t cells, hipStream_t s) {
+    hipLaunchKernelGGL((k_inplace<NS>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, f, cells);
 }
 
-void launch_inplace_n(int slots, float *f, int64_t stride, int64_t cells, hipStream_t s) {
+void launch_inplace_n(int slots, float *f, int64_t cells, hipStream_t s) {
     switch (slots) {
-    case 5: launch_inplace<5>(f, stride, cells, s); break;
-    case 7: launch_inplace<7>(f, stride, cells, s); break;
-    case 9: launch_inplace<9>(f, stride, cells, s); break;
-    case 15: launch_inplace<15>(f, stride, cells, s); break;
-    case 19: launch_inplace<19>(f, stride, cells, s); break;
-    case 27: launch_inplace<27>(f, stride, cells, s); break;
+    case 5: launch_inplace<5>(f, cells, s); break;
+    case 7: launch_inplace<7>(f, cells, s); break;
+    case 9: launch_inplace<9>(f, cells, s); break;
+    case 15: launch_inplace<15>(f, cells, s); break;
+    case 19: launch_inplace<19>(f, cells, s); break;
+    case 27: launch_inplace<27>(f, cells, s); break;
     default: break;
     }
 }
@@ -109,15 +112,16 @@ extern "C" bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, i
     if (!(slots == 5 || slots == 7 || slots == 9 || slots == 15 || slots == 19 || slots == kMaxSlots))
         return BF_ERR_INVALID;
     if (hipSetDevice(device) != hipSuccess) return BF_ERR_DEVICE;
-    const int64_t stride = (cells + 63) / 64 * 64;
-    const size_t bytes = sizeof(float) * (size_t)stride * slots;
+    // whole slabs: the last one is partly used
+    const int64_t P = int64_t(1) << bf::kSlabLog;
+    const size_t bytes = sizeof(float) * (size_t)((cells + P - 1) / P * P) * slots;
     float *f = nullptr;
     if (hipMalloc((void **)&f, bytes) != hipSuccess) return BF_ERR_OOM;
     float ms = 0.0f;
     bf_status st = BF_OK;
     if (hipMemset(f, 0, bytes) != hipSuccess) st = BF_ERR_DEVICE;
     if (st == BF_OK)
-        st = time_launches([&](hipStream_t s) { launch_inplace_n(slots, f, stride, cells, s); }, iters, &ms);
+        st = time_launches([&](hipStream_t s) { launch_inplace_n(slots, f, cells, s); }, iters, &ms);
     if (st == BF_OK) *gbps = 2.0 * sizeof(float) * (double)cells * slots * iters / (ms * 1e-3) / 1e9;
     (void)hipFree(f);
     return st;

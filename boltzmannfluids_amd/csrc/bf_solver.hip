@@ -29,7 +29,12 @@ struct bf_solver {
     bf_config cfg{};
     int vs = 0, q = 0, qt = 0;
     int X = 0, Y = 0, Z = 0;
-    int64_t N = 0, stride = 0;
+    int64_t N = 0, stride = 0;  // stride: thermal slots (and lattice slots without slabs)
+    // lattice layout (described with StepArgs in bf_kernels.h)
+    uint32_t slab_log = 31, slab_skip = 0, pair_step = 0;
+    int group_width = 0, groups = 0;  // 0 groups: plain structure of arrays
+    size_t slot_off[kMaxQ] = {};      // byte offset of slot i from f
+    size_t lattice_bytes = 0;
     int storage = ST_FP32;
     size_t elem = 4;
     hipStream_t stream = nullptr;
@@ -68,11 +73,17 @@ struct bf_solver {
 
 static std::string g_create_error;
 
-// Slot stride: the cell count rounded up to 64 elements (256-byte aligned
-// slots) plus a pad of 16384 elements, which places the slot bases at
-// different offsets modulo the memory interleave (measured 0.6-2 % faster on
-// the 256^3 and 384^3 D3Q19 workloads).
+// Slot stride of the thermal lattice and of a lattice without slabs: the cell
+// count rounded up to 64 elements (256-byte aligned slots) plus a pad of 16384
+// elements, which places the slot bases at different offsets modulo the
+// memory interleave (measured 0.6-2 % faster than no pad for 19 plain slots).
 constexpr int64_t kSlotPad = 16384;
+// Lattice slabs of 2^kSlabLog cells.  With all 19 slots of a slab in one
+// region (608 KB in FP32) instead of 19 arrays a lattice apart, an in-place
+// pass over 19 slots moves 6.4-6.5 instead of 6.0 TB/s at 256^3 and 6.3-6.4
+// instead of 5.5 TB/s at 384^3 (measured on the MI355X for 2^12 to 2^14
+// cells; 2^18 and more approach the plain arrays again).  2^13 (kSlabLog)
+// gave the fastest tick.
 // Largest grid: two 16-bit slots, (2 N + 63 + kSlotPad) elements, stay below
 // 4 GiB of 32-bit byte offsets.
 constexpr int64_t kMaxCells = (int64_t(1) << 30) - (int64_t(1) << 14);
@@ -106,6 +117,65 @@ bf_status dalloc(bf_solver *s, T **p, size_t bytes) {
 
 void dfree(void *p) {
     if (p) (void)hipFree(p);
+}
+
+// Lattice layout: slabs of 2^kSlabLog cells, the slots in the fewest groups
+// of equal width whose arrays stay below 4 GiB (the kernels address a slot
+// with 32-bit byte offsets from its base).  Group 0 holds the rest slot and
+// the first opposite pairs, each further group the next pairs; a group's
+// unused positions are padding.  Grids that need more than 4 GiB even for one
+// pair per group keep plain slot arrays.
+void choose_layout(bf_solver *s) {
+    const int64_t P = int64_t(1) << kSlabLog;
+    const int64_t nslabs = (s->N + P - 1) / P;
+    const int pairs = (s->q - 1) / 2;
+    for (int G = 1; G <= pairs + 1; ++G) {
+        int W = 2;
+        while ((W - 1) / 2 + (G - 1) * (W / 2) < pairs) ++W;
+        const uint64_t region = (uint64_t)nslabs * P * W * s->elem;
+        if (region > (uint64_t(1) << 32)) continue;
+        s->groups = G;
+        s->group_width = W;
+        s->slab_log = kSlabLog;
+        s->slab_skip = (uint32_t)((W - 1) * P);
+        s->pair_step = (uint32_t)P;
+        s->lattice_bytes = (size_t)G * region;
+        int g = 0, pos = 1;
+        s->slot_off[0] = 0;
+        for (int j = 0; j < pairs; ++j) {
+            if (pos + 2 > W) { ++g; pos = 0; }
+            for (int h = 0; h < 2; ++h, ++pos)
+                s->slot_off[1 + 2 * j + h] = (size_t)g * region + (size_t)pos * P * s->elem;
+        }
+        return;
+    }
+    s->groups = 0;
+    s->group_width = 0;
+    s->slab_log = 31;
+    s->slab_skip = 0;
+    s->pair_step = (uint32_t)s->stride;
+    s->lattice_bytes = (size_t)s->q * s->stride * s->elem;
+    for (int i = 0; i < s->q; ++i) s->slot_off[i] = (size_t)i * s->stride * s->elem;
+}
+
+// Copies slot i between the device layout and N contiguous host elements.
+hipError_t copy_slot(const bf_solver *s, int i, void *host, bool to_host) {
+    char *dev = static_cast<char *>(s->f) + s->slot_off[i];
+    const size_t e = s->elem, N = (size_t)s->N;
+    if (s->groups == 0)
+        return to_host ? hipMemcpy(host, dev, N * e, hipMemcpyDeviceToHost)
+                       : hipMemcpy(dev, host, N * e, hipMemcpyHostToDevice);
+    const size_t P = size_t(1) << s->slab_log, pitch = (size_t)s->group_width * P * e;
+    const size_t full = N / P, tail = N % P;
+    char *h = static_cast<char *>(host);
+    hipError_t r = hipSuccess;
+    if (full)
+        r = to_host ? hipMemcpy2D(h, P * e, dev, pitch, P * e, full, hipMemcpyDeviceToHost)
+                    : hipMemcpy2D(dev, pitch, h, P * e, P * e, full, hipMemcpyHostToDevice);
+    if (r == hipSuccess && tail)
+        r = to_host ? hipMemcpy(h + full * P * e, dev + full * pitch, tail * e, hipMemcpyDeviceToHost)
+                    : hipMemcpy(dev + full * pitch, h + full * P * e, tail * e, hipMemcpyHostToDevice);
+    return r;
 }
 
 template <int VS>
@@ -186,7 +256,10 @@ StepArgs make_args(bf_solver *s, bool update) {
     a.f = s->f;
     a.g = s->g;
     a.stride = s->stride;
-    for (int i = 0; i < s->q; ++i) a.slot[i] = static_cast<char *>(s->f) + (size_t)i * s->stride * s->elem;
+    for (int i = 0; i < s->q; ++i) a.slot[i] = static_cast<char *>(s->f) + s->slot_off[i];
+    a.slab_log = s->slab_log;
+    a.slab_skip = s->slab_skip;
+    a.pair_step = s->pair_step;
     if (s->g)
         for (int q = 0; q < s->qt; ++q) a.gslot[q] = s->g + (size_t)q * s->stride;
     a.X = s->X; a.Y = s->Y; a.Z = s->Z;
@@ -338,6 +411,7 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
     s->stride = (N + 63) / 64 * 64 + kSlotPad;
     s->storage = cfg->storage;
     s->elem = cfg->storage == BF_STORE_FP32 ? 4 : 2;
+    choose_layout(s);
     s->tau = cfg->relaxation_time;
     for (int i = 0; i < 256; ++i) {
         s->objects_h[i] = ObjectEntry{0.0f, 0.0f, 0.0f, 1.0f};  // referance_temperature (LBM.h:30)
@@ -355,7 +429,7 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
         return bail(BF_ERR_DEVICE);
     }
     bf_status st;
-    if ((st = dalloc(s, &s->f, (size_t)q * s->stride * s->elem)) != BF_OK) return bail(st);
+    if ((st = dalloc(s, &s->f, s->lattice_bytes)) != BF_OK) return bail(st);
     if ((st = dalloc(s, &s->vd, sizeof(float4) * N)) != BF_OK) return bail(st);
     if ((st = dalloc(s, &s->ft, sizeof(float4) * N)) != BF_OK) return bail(st);
     if (hipMemsetAsync(s->vd, 0, sizeof(float4) * N, s->stream) != hipSuccess ||
@@ -514,7 +588,9 @@ bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const f
     if (e != hipSuccess) { cleanup(); return fail(s, BF_ERR_DEVICE, "[LBM Error] upload failed: %s", hipGetErrorString(e)); }
 
     InitArgs ia{};
-    ia.f = s->f; ia.g = s->g; ia.stride = s->stride; ia.n = N;
+    for (int i = 0; i < s->q; ++i) ia.slot[i] = static_cast<char *>(s->f) + s->slot_off[i];
+    ia.slab_log = s->slab_log; ia.slab_skip = s->slab_skip;
+    ia.g = s->g; ia.stride = s->stride; ia.n = N;
     ia.rho = rho_d; ia.u = u_d; ia.temperature = t_d; ia.ids = s->ids;
     ia.objects = s->objects_d; ia.rho_eff = s->rho_eff_d;
     ia.psi0 = s->psi[0]; ia.psi1 = s->psi[1];
@@ -697,8 +773,7 @@ bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes) {
         float *out = static_cast<float *>(dst);
         if (s->storage == ST_FP32) {
             for (int i = 0; i < s->q; ++i) {
-                BF_HIP(s, hipMemcpy(out + i * N, static_cast<float *>(s->f) + (size_t)i * s->stride, sizeof(float) * N,
-                                    hipMemcpyDeviceToHost));
+                BF_HIP(s, copy_slot(s, i, out + i * N, true));
                 const float w = weight_of(s->vs, i);
                 for (size_t c = 0; c < N; ++c) out[i * N + c] += w;
             }
@@ -706,8 +781,7 @@ bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes) {
         }
         std::vector<uint16_t> tmp(N);
         for (int i = 0; i < s->q; ++i) {
-            BF_HIP(s, hipMemcpy(tmp.data(), static_cast<uint16_t *>(s->f) + (size_t)i * s->stride, 2 * N,
-                                hipMemcpyDeviceToHost));
+            BF_HIP(s, copy_slot(s, i, tmp.data(), true));
             const float w = weight_of(s->vs, i);
             for (size_t c = 0; c < N; ++c) {
                 float v;
@@ -770,7 +844,7 @@ bf_status bf_get_info(const bf_solver *s, bf_info *o) {
     o->multiphase = s->cfg.multiphase;
     o->has_boundaries = s->has_ids;
     o->cells = s->N;
-    o->stride = s->stride;
+    o->stride = s->pair_step;
     o->device_bytes = s->device_bytes;
     return BF_OK;
 }
@@ -921,11 +995,9 @@ bf_status write_state(bf_solver *s, const char *dir, uint32_t kind) {
         return BF_OK;
     };
     if (kind == kKindMeso) {
-        // populations slot by slot (the padding between slots is dropped)
+        // populations slot by slot, N cells each (the layout's padding is dropped)
         std::vector<uint8_t> pops(s->q * N * s->elem);
-        for (int i = 0; i < s->q; ++i)
-            BF_HIP(s, hipMemcpy(pops.data() + i * N * s->elem, static_cast<char *>(s->f) + i * (size_t)s->stride * s->elem,
-                                N * s->elem, hipMemcpyDeviceToHost));
+        for (int i = 0; i < s->q; ++i) BF_HIP(s, copy_slot(s, i, pops.data() + i * N * s->elem, true));
         ok = ok && put(file.f, kTagPopulations, pops.data(), pops.size());
         if (s->thermal) {
             std::vector<uint8_t> th(s->qt * N * sizeof(float));
@@ -1052,9 +1124,7 @@ bf_status bf_load_mesoscopic(bf_solver *s, const char *directory, int64_t tick) 
             BF_HIP(s, hipMemcpy(s->g + (size_t)k * s->stride, th.data() + k * N * sizeof(float), N * sizeof(float),
                                 hipMemcpyHostToDevice));
     }
-    for (int i = 0; i < s->q; ++i)
-        BF_HIP(s, hipMemcpy(static_cast<char *>(s->f) + i * (size_t)s->stride * s->elem, pops.data() + i * N * s->elem,
-                            N * s->elem, hipMemcpyHostToDevice));
+    for (int i = 0; i < s->q; ++i) BF_HIP(s, copy_slot(s, i, pops.data() + i * N * s->elem, false));
     if (h.multiphase) {
         BF_HIP(s, hipMemcpy(s->psi[0], ps.data(), N * sizeof(float), hipMemcpyHostToDevice));
         BF_HIP(s, hipMemcpy(s->psi[1], ps.data() + N * sizeof(float), N * sizeof(float), hipMemcpyHostToDevice));

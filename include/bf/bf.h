@@ -8,8 +8,10 @@
  * HIP stream; it is thread-affine (one thread at a time per handle).
  *
  * Semantics of one tick (all velocity sets, all storage types):
- *   - the lattice is stored in place, one array per direction
- *     (structure of arrays, f[i * stride + cell], cell = z*X*Y + y*X + x), and
+ *   - the lattice is stored in place, one slot per direction; the device
+ *     layout is internal (slabs of 8192 cells holding their slots one after
+ *     the other, cell = z*X*Y + y*X + x), and population reads and checkpoint
+ *     files use the logical order population * cells + cell.  The lattice
  *     is updated with the esoteric "AA" pattern of
  *     BoltzmannFluids/Source/GLSL/LBM/collide.comp:244-369: on even ticks a
  *     cell reads slot i at x - c_i and writes slot opp(i) at x + c_i, on odd
@@ -241,7 +243,7 @@ typedef struct bf_info {
     int32_t multiphase;
     int32_t has_boundaries;
     int64_t cells;
-    int64_t stride;            /* elements between two population slots */
+    int64_t stride;            /* elements between slot i and slot i + 1 of one cell */
     int64_t device_bytes;      /* bytes of device memory the handle holds */
 } bf_info;
 
@@ -392,7 +394,7 @@ bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t iters, 
 /* In-place streaming probe (no counterpart in the reference): the memory
  * traffic of one lattice tick without its arithmetic — every cell reads and
  * rewrites one FP32 element in each of `slots` (5, 7, 9, 15, 19 or 27)
- * structure-of-arrays slots of `cells` elements.  *gbps = read + write bytes
+ * slots of `cells` cells in the lattice's slab layout.  *gbps = read + write bytes
  * per second over `iters` launches: the ceiling of the tick's access pattern. */
 bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, int32_t slots, int32_t iters, double *gbps);
 
