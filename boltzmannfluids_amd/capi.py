@@ -103,6 +103,7 @@ def load_library(path=None):
         "bf_synchronize": (i32, [vp]),
         "bf_read_field": (i32, [vp, i32, vp, C.c_size_t]),
         "bf_field_bytes": (C.c_size_t, [vp, i32]),
+        "bf_copy_field_to_device": (i32, [vp, i32, vp, C.c_size_t]),
         "bf_conservation": (i32, [vp, C.POINTER(C.c_double)]),
         "bf_last_step_sums": (i32, [vp, C.POINTER(C.c_double)]),
         "bf_get_info": (i32, [vp, C.POINTER(Info)]),
@@ -409,6 +410,15 @@ class Solver:
         sc = np.ascontiguousarray(mesh_scale, np.float32)
         return _mc_run(lambda vp_, vc, ip, ic, nv, ni: self._lib.bf_isosurface(
             self._h, float(threshold), _fptr(sc), vp_, vc, ip, ic, nv, ni))
+
+    def copy_field_to_device(self, field, address, nbytes):
+        """Copies a field in the encoding of the reference's texture for it
+        (velocity_density / force_temperature: float4 per cell; boundary: one
+        byte per cell, 0 fluid and 255 object) into device memory at `address`,
+        device to device, on the solver's stream."""
+        fid = FIELDS.get(field, field)
+        self._check(self._lib.bf_copy_field_to_device(self._h, fid, C.c_void_p(address), nbytes),
+                    "bf_copy_field_to_device")
 
     def read(self, field):
         fid = FIELDS.get(field, field)

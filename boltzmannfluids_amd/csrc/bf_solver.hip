@@ -1202,3 +1202,50 @@ LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cpt, int vs) {
 }
 
 }  // namespace bf
+
+// ---- field textures: device-to-device copies ------------------------------
+//
+// The reference keeps its macroscopic fields in GL 3-D textures that the
+// collide shader fills with imageStore (collide.comp:158-163,784-792; formats
+// LBM.h:299-301, created in LBM.cpp:957-1003) and that its renderers sample.
+// The MI355X has no image support (HIP arrays are not supported on gfx950)
+// and no graphics pipeline, so no GL texture can live in or share its memory;
+// a device consumer of the fields gets them in the texture encoding by a
+// device-to-device copy instead.
+
+namespace {
+
+// R8 (unsigned normalised) value of imageStore(boundry_texture, vec4(id)):
+// the store clamps to [0, 1], so fluid is 0 and every object 255.
+__global__ void __launch_bounds__(256) k_boundary_r8(const uint8_t *__restrict__ ids, uint8_t *__restrict__ out,
+                                                     int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c < n) out[c] = ids && ids[c] != 0 ? 255 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+bf_status bf_copy_field_to_device(bf_solver *s, int32_t field, void *dst, size_t bytes) {
+    if (!s || !dst) return BF_ERR_INVALID;
+    if (!s->initialized) return fail(s, BF_ERR_STATE, "[LBM Error] field copy before bf_init_fields");
+    size_t need = 0;
+    if (field == BF_FIELD_VELOCITY_DENSITY || field == BF_FIELD_FORCE_TEMPERATURE) need = sizeof(float4) * (size_t)s->N;
+    else if (field == BF_FIELD_BOUNDARY) need = (size_t)s->N;
+    else return fail(s, BF_ERR_INVALID, "[LBM Error] field %d has no texture", field);
+    if (bytes != need)
+        return fail(s, BF_ERR_INVALID, "[LBM Error] field copy: %zu bytes given, the field has %zu", bytes, need);
+    BF_HIP(s, hipSetDevice(s->cfg.device));
+    if (field == BF_FIELD_BOUNDARY) {
+        hipLaunchKernelGGL(k_boundary_r8, dim3((unsigned)((s->N + 255) / 256)), dim3(256), 0, s->stream, s->ids,
+                           static_cast<uint8_t *>(dst), s->N);
+        BF_HIP(s, hipGetLastError());
+        return BF_OK;
+    }
+    const void *src = field == BF_FIELD_VELOCITY_DENSITY ? (const void *)s->vd : (const void *)s->ft;
+    BF_HIP(s, hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToDevice, s->stream));
+    return BF_OK;
+}
+
+}  // extern "C"

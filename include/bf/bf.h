@@ -119,8 +119,9 @@ typedef enum bf_field {
     BF_FIELD_BOUNDARY = 1,
     /* float4 per cell (F.x, F.y, F.z, T); force_temperature_texture (collide.comp:790). */
     BF_FIELD_FORCE_TEMPERATURE = 2,
-    /* Q*N floats: the in-place lattice exactly as stored (AA slot layout,
-     * population-major), converted to unshifted FP32 (h + w in FP32). */
+    /* Q*N floats: the in-place lattice's slots (the AA state, not the
+     * streamed populations), population-major in cell order, converted to
+     * unshifted FP32 (h + w in FP32). */
     BF_FIELD_POPULATIONS_RAW = 3,
     /* Qt*N floats: the in-place thermal lattice as stored (AA slot layout). */
     BF_FIELD_THERMAL_RAW = 4,
@@ -151,8 +152,8 @@ bf_status bf_velocity_table(int32_t set, int32_t thermal, float *out, int32_t ca
 /* Create a solver: validates the config and allocates the lattice.
  * Replaces the allocation half of LBM::initialize_fields
  * (LBM.cpp:415-462 and _generate_lattice_buffer, LBM.cpp:808-876).
- * At most 2^30 - 2^14 cells (each pair of opposite 16-bit population slots,
- * and each FP32 slot, stays below 4 GiB). */
+ * At most 2^30 - 2^14 cells (the lattice is split into slot groups that each
+ * stay below 4 GiB of 32-bit offsets). */
 bf_status bf_create(const bf_config *cfg, bf_solver **out);
 
 /* Release everything the handle owns.  Safe on NULL. */
@@ -222,6 +223,23 @@ bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes);
 
 /* Size in bytes of a field (0 if the field does not exist for this solver). */
 size_t bf_field_bytes(const bf_solver *s, int32_t field);
+
+/* --- field textures (device to device) ---------------------------------- */
+
+/* Copy a macroscopic field, in the encoding of the reference's GL texture for
+ * it, into device memory `dst` of `bytes` bytes (exactly the field's size),
+ * asynchronously on the handle's stream.  Replaces the texture side of the
+ * macroscopic save (imageStore in collide.comp:784-792 into the textures of
+ * LBM.cpp:957-1003, formats LBM.h:299-301) for consumers on the device:
+ *   BF_FIELD_VELOCITY_DENSITY, BF_FIELD_FORCE_TEMPERATURE: RGBA32F texels, one
+ *     float4 per cell in cell order (16 bytes per cell);
+ *   BF_FIELD_BOUNDARY: R8 texels, what the reference's imageStore of
+ *     vec4(boundary id) into an r8 image stores: 0 for fluid, 255 for any
+ *     object (the store clamps to [0, 1]).
+ * The MI355X has no image support and no graphics pipeline, so GL textures
+ * cannot share its memory; a renderer on another device reads the fields with
+ * bf_read_field.  BF_ERR_INVALID for another field or size. */
+bf_status bf_copy_field_to_device(bf_solver *s, int32_t field, void *dst, size_t bytes);
 
 /* FP64 conservation sums over fluid cells of the current state:
  * out[0] = sum rho, out[1..3] = sum of sum_i f_i c_i (raw momentum, no

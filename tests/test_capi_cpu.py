@@ -115,3 +115,11 @@ def test_null_handle_is_rejected(bfa):
     assert lib.bf_step(None, 1, 0) == 1
     assert lib.bf_ticks(None) == -1
     lib.bf_destroy(None)
+
+
+
+def test_field_copy_rejects_null_handles(bfa):
+    """bf_copy_field_to_device checks its arguments before any device call (no
+    GPU needed): a null handle or destination is BF_ERR_INVALID (1)."""
+    lib = bfa.load_library()
+    assert lib.bf_copy_field_to_device(None, 0, None, 16) == 1

@@ -68,3 +68,39 @@ def test_droplet_isosurface_through_marching_cubes_shim(tmp_path):
     for v, t in meshes:
         assert np.array_equal(v.view(np.uint32), ov.view(np.uint32)) and np.array_equal(t, ot)
     assert len(ot) > 500
+
+
+
+def test_field_copies_to_device_memory(bfa):
+    """bf_copy_field_to_device: the velocity/density and force/temperature
+    fields arrive in device memory (a torch buffer) bit-identical to the host
+    readback, the boundary field in the reference's R8 texture encoding (0
+    fluid, 255 object); a wrong size or a field without a texture is refused."""
+    import torch
+    X, Y, Z = 40, 24, 12
+    cells = X * Y * Z
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    ids = (((x - 12) ** 2 + (y - 12) ** 2 + (z - 6) ** 2 <= 16) | (y == 0) | (y == Y - 1)).astype(np.uint8).ravel()
+    print(f"case: field copies to device memory, D3Q19 duct {X}x{Y}x{Z}", flush=True)
+    s = bfa.Solver((X, Y, Z), 0.6, periodic=(1, 0, 1), velocity_set="D3Q19")
+    s.init_fields(np.ones(cells, np.float32), np.zeros((cells, 3), np.float32), boundary_id=ids)
+    s.set_param("constant_force", np.array([2e-5, 0, 0], np.float32))
+    s.step(25, save=True)
+    dev = torch.cuda.current_device()
+    for name in ("velocity_density", "force_temperature"):
+        buf = torch.full((cells, 4), float("nan"), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()  # the fill runs on torch's stream, the copy on the solver's
+        s.copy_field_to_device(name, buf.data_ptr(), buf.numel() * 4)
+        s.synchronize()
+        got = buf.cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), s.read(name).view(np.uint32)), name
+    b = torch.full((cells,), 7, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    s.copy_field_to_device("boundary", b.data_ptr(), cells)
+    s.synchronize()
+    assert np.array_equal(b.cpu().numpy(), np.where(ids != 0, 255, 0).astype(np.uint8))
+    with pytest.raises(bfa.BFError):
+        s.copy_field_to_device("boundary", b.data_ptr(), cells - 1)
+    with pytest.raises(bfa.BFError):
+        s.copy_field_to_device("psi", b.data_ptr(), cells)
+    s.close()
