@@ -185,14 +185,15 @@ __host__ __device__ constexpr int x_dir_index(int i) {
 // Marks an LDS entry that no cell wrote (a signalling-NaN pattern that arithmetic never produces).
 constexpr uint32_t kNoValue = 0x7fa5a5a5u;
 
+// Occupancy: the plain and wall kernels are held to 8 waves per SIMD (the
+// register budget then covers 8 resident waves; the compiler spills a few
+// scalar registers to VGPR lanes instead of lowering occupancy): porous duct
+// FP16 +4 %, FP32 +2.4 %, TGV +1-2 %.  The multiphase and thermal kernels are
+// left to the compiler: held to 7 or 8 waves they spill and run at half speed.
 #ifndef BF_WAVES_PER_EU
-#define BF_WAVES_PER_EU 0
+#define BF_WAVES_PER_EU 8
 #endif
-#if BF_WAVES_PER_EU
-#define BF_OCCUPANCY __attribute__((amdgpu_waves_per_eu(BF_WAVES_PER_EU)))
-#else
-#define BF_OCCUPANCY
-#endif
+#define BF_OCCUPANCY __attribute__((amdgpu_waves_per_eu((MP || TH) ? 1 : BF_WAVES_PER_EU)))
 
 // v modulo n in [0, n).  Tile coordinates are inside the grid except in the
 // halo at the grid's edges, so the integer division only runs on those lanes.
