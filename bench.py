@@ -57,12 +57,25 @@ def roofline(bpc, cells, kernel_ms, traffic=None):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
 
 
+# Warm-up covers at least this much device time.  After the GPU idles (the
+# host set-up of each workload), the kernels with the most instructions per
+# byte (the 16-bit even tick) run a power-management transient: fast for ~1 ms,
+# up to 1.6x slower ~4 ms in, settled after ~60 ms; the memory-bound odd tick
+# stays within 15 %.  The timed region starts in the steady state.
+WARMUP_MIN_S = 0.15
+
+
 def timed_run(s, steps, warmup, conservation_every=0):
-    """Warm up, then time `steps` ticks with host wall clock (bracketed by device
-    synchronisation) and HIP events on the solver's stream."""
+    """Warm up (`warmup` ticks and at least WARMUP_MIN_S of running), then time
+    `steps` ticks with host wall clock (bracketed by device synchronisation) and
+    HIP events on the solver's stream."""
     import torch
+    t0 = time.perf_counter()
     s.step(warmup)
     s.synchronize()
+    while time.perf_counter() - t0 < WARMUP_MIN_S:
+        s.step(max(2, warmup) // 2 * 2)   # whole tick pairs: the timed region starts on the same parity
+        s.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if conservation_every:
@@ -328,6 +341,21 @@ def spinodal_sample(bfa, ticks, n=192):
     print(f"spinodal sample: {n}^3, {ticks} ticks", flush=True)
 
 
+def transient_sample(bfa, n=256, chunks=4, ticks=300, pause_s=1.0):
+    """The FP16 256^3 TGV in `chunks` runs of `ticks` ticks with the GPU idle
+    for `pause_s` between them (for a rocprofv3 kernel trace of the
+    power-management transient after idle)."""
+    rho, u = taylor_green(n)
+    s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19", storage="fp16")
+    s.init_fields(rho, u)
+    for k in range(chunks):
+        s.step(ticks)
+        s.synchronize()
+        print(f"transient sample: chunk {k} done", flush=True)
+        time.sleep(pause_s)
+    s.close()
+
+
 def adjoint_sample(bfa, ticks):
     """Short workload for rocprofv3: the D3Q19 128^3 channel forward and its
     gradient over `ticks` ticks with checkpoints only (K = 100)."""
@@ -359,6 +387,8 @@ def main():
                     help="run only TICKS ticks of the 384^3 porous duct per storage type (for rocprofv3)")
     ap.add_argument("--spinodal-sample", type=int, default=0, metavar="TICKS",
                     help="run only TICKS ticks of the 192^3 Shan-Chen + thermal workload (for rocprofv3)")
+    ap.add_argument("--transient-sample", action="store_true",
+                    help="run only the FP16 TGV in chunks with idle pauses (for rocprofv3)")
     ap.add_argument("--adjoint-sample", type=int, default=0, metavar="TICKS",
                     help="run only the D3Q19 128^3 channel gradient over TICKS ticks (for rocprofv3)")
     args = ap.parse_args()
@@ -380,6 +410,9 @@ def main():
     if args.adjoint_sample:
         adjoint_sample(bfa, args.adjoint_sample)
         return
+    if args.transient_sample:
+        transient_sample(bfa)
+        return
     n = args.size
     stores = ["fp32"] if args.fp32_only else ["fp32", "fp16", "bf16"]
     res = {st: run_tgv(bfa, n, st, args.steps, args.warmup) for st in stores}
@@ -391,6 +424,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_min_s": WARMUP_MIN_S,
         "ms_per_step": round(main_run["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
