@@ -84,8 +84,10 @@ constexpr int64_t kSlotPad = 16384;
 // instead of 5.5 TB/s at 384^3 (measured on the MI355X for 2^12 to 2^14
 // cells; 2^18 and more approach the plain arrays again).  2^13 (kSlabLog)
 // gave the fastest tick.
-// Largest grid: two 16-bit slots, (2 N + 63 + kSlotPad) elements, stay below
-// 4 GiB of 32-bit byte offsets.
+// Largest grid: a slot group of one opposite pair of 16-bit slots (slabs of
+// 2 x 8192 elements) and, for FP32 lattices too large for slabs, one plain
+// slot array (N + 63 + kSlotPad elements) stay within 4 GiB of 32-bit byte
+// offsets.
 constexpr int64_t kMaxCells = (int64_t(1) << 30) - (int64_t(1) << 14);
 
 namespace {
@@ -387,9 +389,9 @@ bf_status bf_create(const bf_config *cfg, bf_solver **out) {
     if (!(cfg->relaxation_time > 0.5f))
         return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: relaxation time %g must exceed 0.5", (double)cfg->relaxation_time);
     const int64_t N = (int64_t)X * Y * Z;
-    // each slot (stride cells of up to 4 bytes) and each pair of opposite
-    // 16-bit slots is addressed with 32-bit byte offsets; the reference's
-    // uint32 element addressing (collide.comp:196-200) allows Q * N < 2^32 only
+    // slots are addressed with 32-bit byte offsets from their base (see
+    // kMaxCells); the reference's uint32 element addressing
+    // (collide.comp:196-200) allows Q * N < 2^32 only
     if (N > kMaxCells)
         return fail(nullptr, BF_ERR_INVALID, "[LBM Error] bf_create: %lld cells exceed the limit of %lld",
                     (long long)N, (long long)kMaxCells);
