@@ -112,39 +112,45 @@ def test_two_slot_groups_384_against_oracle(bfa, orc, tmp_path):
     assert same
 
 
-def test_largest_fp32_grid_plain_slot_arrays(bfa):
-    """An FP32 grid above 2^29 cells cannot keep even one opposite pair per
-    4 GiB slot group, so the solver falls back to plain slot arrays (the
-    largest grids it accepts).  D2Q9 32768 x 16385 (2^29 + 32768 cells, a
-    19 GB lattice), periodic, a shear wave u_x(x): the flow does not depend on
-    y, so after 6 ticks every row must equal the same wave run on a 4-row grid
-    (slab layout) bit for bit."""
-    X, Y = 32768, 16385
-    cells = X * Y
-    assert cells > 2 ** 29
+@pytest.mark.parametrize("storage,rows", [("fp32", 16385), ("fp16", 32767)])
+def test_largest_grids(bfa, storage, rows):
+    """The largest grids the solver accepts, on their extreme layouts.
+    - FP32 above 2^29 cells cannot keep even one opposite pair per 4 GiB slot
+      group, so the solver falls back to plain slot arrays: D2Q9 32768 x 16385
+      (2^29 + 32768 cells, a 19 GB lattice).
+    - FP16 just under the 2^30 - 2^14 cell limit needs slot groups of width 2
+      (five groups for D2Q9): D2Q9 32768 x 32767 (a 19 GB lattice).
+    A periodic shear wave u_y(x) with rho(x) does not depend on y, so after 6
+    ticks every row must equal the same wave run on a 4-row grid (one slab
+    group) bit for bit."""
+    X = 32768
+    cells = X * rows
     ticks = 6
     row_u = np.zeros((X, 3), np.float32)
     row_u[:, 1] = (0.05 * np.sin(2 * np.pi * np.arange(X) / X)).astype(np.float32)
     row_rho = (1.0 + 0.01 * np.cos(2 * np.pi * np.arange(X) / X)).astype(np.float32)
 
-    def run(rows):
-        s = bfa.Solver((X, rows, 1), 0.6, periodic=(1, 1, 1), velocity_set="D2Q9")
-        s.init_fields(np.tile(row_rho, rows), np.tile(row_u, (rows, 1)))
+    def run(n_rows):
+        s = bfa.Solver((X, n_rows, 1), 0.6, periodic=(1, 1, 1), velocity_set="D2Q9", storage=storage)
+        s.init_fields(np.tile(row_rho, n_rows), np.tile(row_u, (n_rows, 1)))
         s.step(ticks, save=True)
         return s
 
-    say(f"case: D2Q9 FP32 {X}x{Y} ({cells} cells, plain slot arrays), {ticks} ticks, against a 4-row grid")
+    say(f"case: D2Q9 {storage} {X}x{rows} ({cells} cells), {ticks} ticks, against a 4-row grid")
     small = run(4)
     ref = small.read("velocity_density")[:X].copy()
     small.close()
-    big = run(Y)
+    big = run(rows)
     info = big.info()
-    vd = big.read("velocity_density").reshape(Y, X, 4)
+    vd = big.read("velocity_density").reshape(rows, X, 4)
     big.close()
-    rows_equal = all(np.array_equal(vd[y0:y0 + 1024], np.broadcast_to(ref, (min(1024, Y - y0), X, 4)))
-                     for y0 in range(0, Y, 1024))
+    rows_equal = all(np.array_equal(vd[y0:y0 + 1024], np.broadcast_to(ref, (min(1024, rows - y0), X, 4)))
+                     for y0 in range(0, rows, 1024))
     say(f"  slot stride {info['stride']}; every row equals the 4-row run: {rows_equal}")
-    assert info["stride"] >= cells      # plain slot arrays (slabs report 8192)
+    if storage == "fp32":
+        assert cells > 2 ** 29 and info["stride"] >= cells    # plain slot arrays
+    else:
+        assert cells >= 2 ** 30 - 2 ** 15 and info["stride"] == 8192   # slabs
     assert rows_equal
 
 
