@@ -441,9 +441,18 @@ def main():
     copy = bfa.capi.copy_bandwidth(1 << 30, 20)
     inplace = bfa.capi.inplace_bandwidth(n ** 3, 19, 20)
     inplace_half = bfa.capi.inplace_bandwidth(n ** 3 // 2, 19, 20)
+    # the AA tick's two memory patterns (odd: every slot at the cell; even: the
+    # y/z-shifted rows) at the workload's size; a tick pair takes 1/B_odd +
+    # 1/B_even per byte, so their harmonic mean is the pair's ceiling
+    aa_odd = bfa.capi.aa_bandwidth(n, n, n, False, 20)
+    aa_even = bfa.capi.aa_bandwidth(n, n, n, True, 20)
+    aa_pair = 2.0 / (1.0 / aa_odd + 1.0 / aa_even)
     line["ceiling"] = {"copy_GBs": round(copy, 1), "inplace_19slot_GBs": round(inplace, 1),
                        "inplace_19slot_half_bytes_GBs": round(inplace_half, 1),
-                       "fp32_kernel_frac_of_inplace": round(main_run["roofline"]["achieved"] / inplace, 4)}
+                       "fp32_kernel_frac_of_inplace": round(main_run["roofline"]["achieved"] / inplace, 4),
+                       "aa_odd_GBs": round(aa_odd, 1), "aa_even_GBs": round(aa_even, 1),
+                       "aa_tick_pair_GBs": round(aa_pair, 1),
+                       "fp32_kernel_frac_of_aa_pattern": round(main_run["roofline"]["achieved"] / aa_pair, 4)}
     for st in ("fp16", "bf16"):
         if st in res:
             r = res[st]

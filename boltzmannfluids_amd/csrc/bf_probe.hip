@@ -14,6 +14,7 @@
 
 #include "../../include/bf/bf.h"
 #include "bf_kernels.h"
+#include "bf_lattice.h"
 
 namespace {
 
@@ -55,6 +56,36 @@ void launch_inplace_n(int slots, float *f, int64_t cells, hipStream_t s) {
     case 19: launch_inplace<19>(f, cells, s); break;
     case 27: launch_inplace<27>(f, cells, s); break;
     default: break;
+    }
+}
+
+// The D3Q19 AA pattern on a periodic X x Y x Z grid in the slab layout, one
+// block per x-row: on the odd tick every cell reads and rewrites its 19 slots
+// at the cell itself; on the even tick it reads slot i in row (y - c_iy,
+// z - c_iz) and writes slot opp(i) in row (y + c_iy, z + c_iz), each at its
+// own x (the tick kernel moves the x shift through LDS, so memory sees these
+// rows).  The read set of every cell is its write set, as in the tick.
+template <bool EVEN>
+__global__ void __launch_bounds__(1024) k_aa_pattern(float *__restrict__ f, int X, int Y, int Z) {
+    using L = bf::Lattice<bf::D3Q19>;
+    const int x = threadIdx.x, y = blockIdx.x, z = blockIdx.y;
+    if (x >= X) return;
+    const int ys[3] = {y == 0 ? Y - 1 : y - 1, y, y == Y - 1 ? 0 : y + 1};
+    const int zs[3] = {z == 0 ? Z - 1 : z - 1, z, z == Z - 1 ? 0 : z + 1};
+    constexpr int64_t P = int64_t(1) << bf::kSlabLog;
+    auto el = [&](int i, int yy, int zz) {
+        const int64_t c = x + (int64_t)X * (yy + (int64_t)Y * zz);
+        return ((c >> bf::kSlabLog) * L::Q + i) * P + (c & (P - 1));
+    };
+    float v[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i)
+        v[i] = __builtin_nontemporal_load(f + (EVEN ? el(i, ys[1 - L::c[i][1]], zs[1 - L::c[i][2]]) : el(i, y, z)));
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const int o = bf::opposite(i);
+        const int64_t e = EVEN ? el(o, ys[1 + L::c[i][1]], zs[1 + L::c[i][2]]) : el(o, y, z);
+        __builtin_nontemporal_store(0.5f * v[i] + v[(i + 1) % L::Q], f + e);
     }
 }
 
@@ -123,6 +154,29 @@ extern "C" bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, i
     if (st == BF_OK)
         st = time_launches([&](hipStream_t s) { launch_inplace_n(slots, f, cells, s); }, iters, &ms);
     if (st == BF_OK) *gbps = 2.0 * sizeof(float) * (double)cells * slots * iters / (ms * 1e-3) / 1e9;
+    (void)hipFree(f);
+    return st;
+}
+
+extern "C" bf_status bf_probe_aa_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t even,
+                                           int32_t iters, double *gbps) {
+    if (!gbps || iters < 1 || X < 1 || X > 1024 || Y < 1 || Y > 65535 || Z < 1 || Z > 65535) return BF_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return BF_ERR_DEVICE;
+    const int64_t cells = (int64_t)X * Y * Z;
+    const int64_t P = int64_t(1) << bf::kSlabLog;
+    const size_t bytes = sizeof(float) * (size_t)((cells + P - 1) / P * P) * 19;
+    float *f = nullptr;
+    if (hipMalloc((void **)&f, bytes) != hipSuccess) return BF_ERR_OOM;
+    float ms = 0.0f;
+    bf_status st = BF_OK;
+    if (hipMemset(f, 0, bytes) != hipSuccess) st = BF_ERR_DEVICE;
+    const dim3 grid((unsigned)Y, (unsigned)Z), block((unsigned)((X + 63) / 64 * 64));
+    if (st == BF_OK)
+        st = time_launches([&](hipStream_t s) {
+            if (even) hipLaunchKernelGGL(k_aa_pattern<true>, grid, block, 0, s, f, X, Y, Z);
+            else hipLaunchKernelGGL(k_aa_pattern<false>, grid, block, 0, s, f, X, Y, Z);
+        }, iters, &ms);
+    if (st == BF_OK) *gbps = 2.0 * sizeof(float) * 19.0 * (double)cells * iters / (ms * 1e-3) / 1e9;
     (void)hipFree(f);
     return st;
 }

@@ -416,6 +416,17 @@ bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t iters, 
  * per second over `iters` launches: the ceiling of the tick's access pattern. */
 bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, int32_t slots, int32_t iters, double *gbps);
 
+/* AA-pattern probe (no counterpart in the reference): the memory traffic of
+ * one D3Q19 tick of the in-place AA update on a periodic X x Y x Z grid in the
+ * lattice's slab layout, without arithmetic.  even = 0: every cell reads and
+ * rewrites its 19 slots at the cell itself (the odd tick); even = 1: slot i
+ * is read in the row (y - c_iy, z - c_iz) and slot opp(i) written in the row
+ * (y + c_iy, z + c_iz), at the cell's own x (the even tick, whose x shift goes
+ * through LDS).  X <= 1024.  *gbps = read + write bytes per second over
+ * `iters` launches: the ceiling of each tick's access pattern. */
+bf_status bf_probe_aa_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t even, int32_t iters,
+                                double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,3 +123,15 @@ def test_field_copy_rejects_null_handles(bfa):
     GPU needed): a null handle or destination is BF_ERR_INVALID (1)."""
     lib = bfa.load_library()
     assert lib.bf_copy_field_to_device(None, 0, None, 16) == 1
+
+
+def test_aa_probe_rejects_bad_arguments(bfa):
+    """bf_probe_aa_bandwidth checks its arguments before any device call: rows
+    longer than one block (X > 1024), empty grids and a null result are
+    BF_ERR_INVALID (1)."""
+    import ctypes as C
+    lib = bfa.load_library()
+    out = C.c_double()
+    assert lib.bf_probe_aa_bandwidth(0, 2048, 8, 8, 1, 1, C.byref(out)) == 1
+    assert lib.bf_probe_aa_bandwidth(0, 0, 8, 8, 0, 1, C.byref(out)) == 1
+    assert lib.bf_probe_aa_bandwidth(0, 64, 8, 8, 0, 1, None) == 1
