@@ -1191,7 +1191,8 @@ LaunchShape step_shape(int X, int Y, int Z, bool multiphase, int cpt, int vs) {
             const int bx = row_threads;
             const int by = std::max(1, BF_BLOCK_CELLS / bx);
             sh.block = dim3(bx, by, 1);
-            sh.lds_bytes = BF_ALIGN_ROWS ? sizeof(float) * (size_t)x_direction_count(vs) * by * X : 0;
+            // [row][x][x-direction], the directions at an odd stride
+            sh.lds_bytes = BF_ALIGN_ROWS ? sizeof(float) * (size_t)(x_direction_count(vs) | 1) * by * X : 0;
         } else {
             const int bx = std::min(BF_BLOCK_CELLS, row_threads);
             const int by = std::max(1, BF_BLOCK_CELLS / bx);

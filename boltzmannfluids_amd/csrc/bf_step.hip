@@ -344,9 +344,13 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     float graw[QT];
     float fi[CPT][Q];
     float gi[QT];
-    const int lds_plane = (int)blockDim.y * X;  // row buffer: [x-direction][row of the block][x]
+    // row buffer: [row of the block][x][x-direction], the directions of one
+    // column at an odd stride (lanes of consecutive columns hit different
+    // banks), so that the directions of a column are immediate offsets from
+    // one address
+    constexpr int kXdStride = x_dir_count<VS>() | 1;
     const int lds_base = ty * X;
-#define LDS(i, col) lds_rows[x_dir_index<VS>(i) * lds_plane + lds_base + (col)]
+#define LDS(i, col) lds_rows[(lds_base + (col)) * kXdStride + x_dir_index<VS>(i)]
 
     // 1b.  Cells that do not update (solid, or past the grid) load cell 0 of
     // the same slots: one shared line per wave, and no branch splits the
