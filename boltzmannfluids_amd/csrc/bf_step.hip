@@ -733,7 +733,9 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 const int sx = 1 - L::c[i][0];             // the writer sits at x - c_x
                 if (!(PER || vx[k][sx])) continue;
                 const float v = LDS(i, x);
-                if (__float_as_uint(v) == kNoValue) continue;
+                // (the periodic kernel has no empty entries: every cell of the
+                // grid updates and no write leaves the domain)
+                if (!PER && __float_as_uint(v) == kNoValue) continue;
                 const int dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                 S::store(SLOT(opposite(i)), ROW(k, dy, dz), v);
             }
