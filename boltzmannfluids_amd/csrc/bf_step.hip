@@ -69,13 +69,20 @@ __device__ inline void st(T *p, T v) {
 // 32-bit byte offset: the loads and stores take the scalar-base form and cells
 // that share an index share one offset register.  The host keeps every slot
 // group below 4 GiB, so every element's byte offset from its slot base is 32-bit.
+// The byte offset passes through an empty asm: the compiler otherwise
+// rewrites zext(idx * size) as zext(idx) << log2(size) where idx comes from a
+// select, a 64-bit offset that the scalar-base form cannot take.
 template <class T>
 __device__ inline const T *at(const T *p, uint32_t idx) {
-    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + idx * (uint32_t)sizeof(T));
+    uint32_t off = idx * (uint32_t)sizeof(T);
+    asm("" : "+v"(off));
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + off);
 }
 template <class T>
 __device__ inline T *at(T *p, uint32_t idx) {
-    return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + idx * (uint32_t)sizeof(T));
+    uint32_t off = idx * (uint32_t)sizeof(T);
+    asm("" : "+v"(off));
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + off);
 }
 // Byte offset of element idx with its known bits hidden.  Where one index
 // serves every slot (the odd tick), the compiler otherwise rewrites
