@@ -242,7 +242,10 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #define PAIR_BASE(j) SLOT((j) < opposite(j) ? (j) : opposite(j))
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int y = blockIdx.y * blockDim.y + ty;
+    // PER: lanes past the grid address the grid's last row / column (they
+    // load in range and store nothing), so no access needs a per-lane guard
+    const int y_in = blockIdx.y * blockDim.y + ty;
+    const int y = PER ? min(y_in, Y - 1) : y_in;
     const int z = blockIdx.z;
 
     // Neighbour coordinates: index 0 = -1, 1 = 0, 2 = +1 along each axis.
@@ -268,13 +271,14 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     bool in_bounds[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int x = (blockIdx.x * CPT + k) * blockDim.x + tx;
+        const int x_in = (blockIdx.x * CPT + k) * blockDim.x + tx;
+        const int x = PER ? min(x_in, X - 1) : x_in;
         const bool w0 = x - 1 < 0, w2 = x + 1 >= X;
         xs[k][0] = w0 ? x - 1 + X : x - 1; xs[k][1] = x; xs[k][2] = w2 ? x + 1 - X : x + 1;
         vx[k][0] = !w0 || a.periodic_x != 0; vx[k][1] = true; vx[k][2] = !w2 || a.periodic_x != 0;
         self[k] = x + ry[1] + pz[1];
-        active[k] = x < X && y < Y;
-        in_bounds[k] = active[k];
+        active[k] = x_in < X && y_in < Y;
+        in_bounds[k] = active[k];   // in the grid (active drops solid cells below)
     }
     // AL: every population access of the tick is at the cell's own x in one of
     // the 9 neighbouring rows.  With 16-bit storage their elements are computed
@@ -368,7 +372,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         if constexpr (AL) {
             // every cell of the block's rows loads, at its own x, slot i of the
             // row the cell at x + c_x pulls from
-            const bool in_grid = xs[k][1] < X && y < Y;
+            const bool in_grid = PER || in_bounds[k];
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
@@ -377,7 +381,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 raw[k][i] = S::raw(SLOT(i), idx);
             }
         }
-        const bool act = active[k];
+        const bool act = PER || active[k];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int o = opposite(i);
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                 // every cell of the grid loads its own slots: the addresses do
                 // not wait for the boundary id (a solid cell shares its lines
                 // with the fluid cells of its wave)
-                raw[k][i] = S::raw_b(SLOT(o), opaque_offset(in_bounds[k] ? pself[k] : 0u, sizeof(T)));
+                raw[k][i] = S::raw_b(SLOT(o), opaque_offset((PER || in_bounds[k]) ? pself[k] : 0u, sizeof(T)));
                 continue;
             }
             if (i == 0) {
@@ -435,7 +439,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     if constexpr (AL) {
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
-            if (!(xs[k][1] < X && y < Y)) continue;
+            if (!in_bounds[k]) continue;
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
@@ -504,8 +508,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
         // other cells of the grid mark theirs empty here.
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
-            const int x = xs[k][1];
-            if (x >= X || y >= Y || active[k]) continue;
+            if (!in_bounds[k] || active[k]) continue;
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
@@ -733,7 +736,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int x = xs[k][1];
-            if (x >= X || y >= Y || !a.update) continue;
+            if (!in_bounds[k] || !a.update) continue;
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
