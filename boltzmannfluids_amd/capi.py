@@ -23,7 +23,7 @@ PARAMS = {
     "shan_chen_synchronous": 9,
 }
 FIELDS = {"velocity_density": 0, "boundary": 1, "force_temperature": 2,
-          "populations_raw": 3, "thermal_raw": 4, "psi": 5}
+          "populations_raw": 3, "thermal_raw": 4, "psi": 5, "populations": 6}
 STEP_SAVE_MACRO = 1
 STEP_CONSERVATION = 2
 STATUS = {0: "BF_OK", 1: "BF_ERR_INVALID", 2: "BF_ERR_STATE", 3: "BF_ERR_UNSUPPORTED",
@@ -446,7 +446,7 @@ class Solver:
         n = self.cells
         if fid in (0, 2):
             return out.reshape(n, 4)
-        if fid in (3, 4):
+        if fid in (3, 4, 6):
             return out.reshape(-1, n)
         return out
 

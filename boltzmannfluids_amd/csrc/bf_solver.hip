@@ -734,9 +734,66 @@ size_t bf_field_bytes(const bf_solver *s, int32_t field) {
     case BF_FIELD_POPULATIONS_RAW: return sizeof(float) * (size_t)s->q * s->N;
     case BF_FIELD_THERMAL_RAW: return s->thermal ? sizeof(float) * (size_t)s->qt * s->N : 0;
     case BF_FIELD_PSI: return s->cfg.multiphase ? sizeof(float) * (size_t)s->N : 0;
+    case BF_FIELD_POPULATIONS: return sizeof(float) * (size_t)s->q * s->N;
     }
     return 0;
 }
+
+namespace {
+
+// The populations each cell's next collision uses, from the slots (unshifted
+// FP32, population-major) by the AA read rules of that tick, as the tick
+// kernel applies them: odd tick slot opp(i) at the cell; even tick slot i at
+// x - c_i, 0.01 outside the domain (collide.comp:297-298), and with halfway
+// bounce-back the own slot opp(i) plus 6 w_i (c_i . u_w) where x - c_i is
+// solid.  Solid cells keep their slots as stored.
+void logical_populations(const bf_solver *s, const std::vector<float> &raw, const std::vector<uint8_t> &ids,
+                         bool odd, float *out) {
+    float tab[4 * kMaxQ];
+    bf_velocity_table(s->vs, 0, tab, 4 * kMaxQ);
+    const int64_t N = s->N;
+    const int X = s->X, Y = s->Y, Z = s->Z;
+    const bool halfway = s->cfg.boundary_mode == BF_BB_HALFWAY;
+    const int per[3] = {s->cfg.periodic[0] != 0, s->cfg.periodic[1] != 0, s->cfg.periodic[2] != 0};
+    for (int64_t c = 0; c < N; ++c) {
+        const int x = (int)(c % X), y = (int)((c / X) % Y), z = (int)(c / ((int64_t)X * Y));
+        for (int i = 0; i < s->q; ++i) {
+            const int o = opposite(i);
+            float v;
+            if (ids[c] != 0) {
+                v = raw[(size_t)i * N + c];
+            } else if (odd) {
+                v = raw[(size_t)o * N + c];
+            } else if (i == 0) {
+                v = raw[c];
+            } else {
+                int n3[3] = {x - (int)tab[4 * i], y - (int)tab[4 * i + 1], z - (int)tab[4 * i + 2]};
+                const int ext[3] = {X, Y, Z};
+                bool inside = true;
+                for (int d = 0; d < 3; ++d) {
+                    if (n3[d] >= 0 && n3[d] < ext[d]) continue;
+                    if (per[d]) n3[d] = (n3[d] + ext[d]) % ext[d];
+                    else inside = false;
+                }
+                if (!inside) {
+                    v = 0.01f;
+                } else {
+                    const int64_t n = n3[0] + (int64_t)X * (n3[1] + (int64_t)Y * n3[2]);
+                    if (halfway && ids[n] != 0) {
+                        const ObjectEntry &ob = s->objects_h[ids[n]];
+                        const float cu = tab[4 * i] * ob.ux + tab[4 * i + 1] * ob.uy + tab[4 * i + 2] * ob.uz;
+                        v = raw[(size_t)o * N + c] + 6.0f * tab[4 * i + 3] * cu;
+                    } else {
+                        v = raw[(size_t)i * N + n];
+                    }
+                }
+            }
+            out[(size_t)i * N + c] = v;
+        }
+    }
+}
+
+}  // namespace
 
 // Device view of the saved density (the w channel of the velocity_density
 // field) for the iso-surface pass (bf_mc.hip).
@@ -803,6 +860,15 @@ bf_status bf_read_field(bf_solver *s, int32_t field, void *dst, size_t bytes) {
     case BF_FIELD_PSI:
         BF_HIP(s, hipMemcpy(dst, s->psi[s->cur ^ 1], need, hipMemcpyDeviceToHost));
         return BF_OK;
+    case BF_FIELD_POPULATIONS: {
+        std::vector<float> raw((size_t)s->q * N);
+        const bf_status st = bf_read_field(s, BF_FIELD_POPULATIONS_RAW, raw.data(), raw.size() * sizeof(float));
+        if (st != BF_OK) return st;
+        std::vector<uint8_t> ids(N, 0);
+        if (s->ids) BF_HIP(s, hipMemcpy(ids.data(), s->ids, N, hipMemcpyDeviceToHost));
+        logical_populations(s, raw, ids, ((s->ticks - s->tick_base) & 1) != 0, static_cast<float *>(dst));
+        return BF_OK;
+    }
     }
     return BF_ERR_INVALID;
 }

@@ -126,7 +126,16 @@ typedef enum bf_field {
     /* Qt*N floats: the in-place thermal lattice as stored (AA slot layout). */
     BF_FIELD_THERMAL_RAW = 4,
     /* N floats: pseudopotential psi written on the last tick (Shan-Chen). */
-    BF_FIELD_PSI = 5
+    BF_FIELD_PSI = 5,
+    /* Q*N floats, population-major: the populations f_i(x) that the next
+     * tick's collision at x uses (after streaming, before collision), derived
+     * from the slots by that tick's AA read rules (collide.comp:244-305): on
+     * an odd tick slot opp(i) at x; on an even tick slot i at x - c_i, 0.01
+     * where x - c_i lies outside the domain, and with halfway bounce-back the
+     * cell's own slot opp(i) plus the moving-wall term where x - c_i is
+     * solid.  Solid cells report their slots as stored.  Computed on the
+     * host from BF_FIELD_POPULATIONS_RAW. */
+    BF_FIELD_POPULATIONS = 6
 } bf_field;
 
 /* Flags for bf_step. */
