@@ -458,7 +458,7 @@ def test_conservation_sums_fp64(bfa, orc):
 
 
 def test_full_size_tgv_256_few_ticks(bfa, orc):
-    """BASELINE workload size: 256^3 D3Q19 FP32, indexing checked against the
+    """Headline workload size: 256^3 D3Q19 FP32, indexing checked against the
     FP32 oracle for 3 ticks, then 200 more ticks checked through size-independent
     properties (mass drift, viscous decay rate)."""
     n = 256
