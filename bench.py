@@ -447,11 +447,15 @@ def main():
     aa_odd = bfa.capi.aa_bandwidth(n, n, n, False, 20)
     aa_even = bfa.capi.aa_bandwidth(n, n, n, True, 20)
     aa_pair = 2.0 / (1.0 / aa_odd + 1.0 / aa_even)
+    # the two-lattice pull pattern (y/z-shifted reads, local writes to a second
+    # lattice) at the same size, for comparison with the AA tick pair
+    pull = bfa.capi.pull_bandwidth(n, n, n, 20)
     line["ceiling"] = {"copy_GBs": round(copy, 1), "inplace_19slot_GBs": round(inplace, 1),
                        "inplace_19slot_half_bytes_GBs": round(inplace_half, 1),
                        "fp32_kernel_frac_of_inplace": round(main_run["roofline"]["achieved"] / inplace, 4),
                        "aa_odd_GBs": round(aa_odd, 1), "aa_even_GBs": round(aa_even, 1),
                        "aa_tick_pair_GBs": round(aa_pair, 1),
+                       "two_lattice_pull_GBs": round(pull, 1),
                        "fp32_kernel_frac_of_aa_pattern": round(main_run["roofline"]["achieved"] / aa_pair, 4)}
     for st in ("fp16", "bf16"):
         if st in res:

@@ -113,6 +113,7 @@ def load_library(path=None):
         "bf_probe_copy_bandwidth": (i32, [i32, i64, i32, C.POINTER(C.c_double)]),
         "bf_probe_inplace_bandwidth": (i32, [i32, i64, i32, i32, C.POINTER(C.c_double)]),
         "bf_probe_aa_bandwidth": (i32, [i32, i32, i32, i32, i32, i32, C.POINTER(C.c_double)]),
+        "bf_probe_pull_bandwidth": (i32, [i32, i32, i32, i32, i32, C.POINTER(C.c_double)]),
         "bf_save_mesoscopic": (i32, [vp, C.c_char_p]),
         "bf_save_macroscopic": (i32, [vp, C.c_char_p]),
         "bf_load_mesoscopic": (i32, [vp, C.c_char_p, i64]),
@@ -231,6 +232,19 @@ def aa_bandwidth(X, Y, Z, even, iters=20, device=0):
     st = lib.bf_probe_aa_bandwidth(int(device), int(X), int(Y), int(Z), int(bool(even)), int(iters), C.byref(out))
     if st:
         raise BFError(st, "bf_probe_aa_bandwidth")
+    return out.value
+
+
+def pull_bandwidth(X, Y, Z, iters=20, device=0):
+    """GB/s (read + write) of the two-lattice pull pattern for D3Q19 on a
+    periodic X x Y x Z grid in the slab layout, without arithmetic: slot i read
+    from one lattice in row (y - c_iy, z - c_iz), written to a second lattice
+    at the cell itself."""
+    lib = load_library()
+    out = C.c_double()
+    st = lib.bf_probe_pull_bandwidth(int(device), int(X), int(Y), int(Z), int(iters), C.byref(out))
+    if st:
+        raise BFError(st, "bf_probe_pull_bandwidth")
     return out.value
 
 

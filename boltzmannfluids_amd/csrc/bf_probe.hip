@@ -89,6 +89,31 @@ __global__ void __launch_bounds__(1024) k_aa_pattern(float *__restrict__ f, int 
     }
 }
 
+// The D3Q19 two-lattice pull pattern (for comparison with the AA pattern
+// above): slot i is read from lattice `a` in row (y - c_iy, z - c_iz) and
+// written to lattice `b` at the cell itself, each at its own x.  Launches
+// alternate the two lattices, so every launch reads what the last one wrote.
+__global__ void __launch_bounds__(1024) k_pull_pattern(const float *__restrict__ a, float *__restrict__ b, int X,
+                                                       int Y, int Z) {
+    using L = bf::Lattice<bf::D3Q19>;
+    const int x = threadIdx.x, y = blockIdx.x, z = blockIdx.y;
+    if (x >= X) return;
+    const int ys[3] = {y == 0 ? Y - 1 : y - 1, y, y == Y - 1 ? 0 : y + 1};
+    const int zs[3] = {z == 0 ? Z - 1 : z - 1, z, z == Z - 1 ? 0 : z + 1};
+    constexpr int64_t P = int64_t(1) << bf::kSlabLog;
+    auto el = [&](int i, int yy, int zz) {
+        const int64_t c = x + (int64_t)X * (yy + (int64_t)Y * zz);
+        return ((c >> bf::kSlabLog) * L::Q + i) * P + (c & (P - 1));
+    };
+    float v[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i)
+        v[i] = __builtin_nontemporal_load(a + el(i, ys[1 - L::c[i][1]], zs[1 - L::c[i][2]]));
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i)
+        __builtin_nontemporal_store(0.5f * v[i] + v[(i + 1) % L::Q], b + el(i, y, z));
+}
+
 // Times `iters` launches of `launch` after one warm-up launch on a private stream.
 template <class F>
 bf_status time_launches(F launch, int32_t iters, float *ms) {
@@ -178,5 +203,32 @@ extern "C" bf_status bf_probe_aa_bandwidth(int32_t device, int32_t X, int32_t Y,
         }, iters, &ms);
     if (st == BF_OK) *gbps = 2.0 * sizeof(float) * 19.0 * (double)cells * iters / (ms * 1e-3) / 1e9;
     (void)hipFree(f);
+    return st;
+}
+
+extern "C" bf_status bf_probe_pull_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t iters,
+                                             double *gbps) {
+    if (!gbps || iters < 1 || X < 1 || X > 1024 || Y < 1 || Y > 65535 || Z < 1 || Z > 65535) return BF_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return BF_ERR_DEVICE;
+    const int64_t cells = (int64_t)X * Y * Z;
+    const int64_t P = int64_t(1) << bf::kSlabLog;
+    const size_t bytes = sizeof(float) * (size_t)((cells + P - 1) / P * P) * 19;
+    float *a = nullptr, *b = nullptr;
+    if (hipMalloc((void **)&a, bytes) != hipSuccess) return BF_ERR_OOM;
+    if (hipMalloc((void **)&b, bytes) != hipSuccess) { (void)hipFree(a); return BF_ERR_OOM; }
+    float ms = 0.0f;
+    bf_status st = BF_OK;
+    if (hipMemset(a, 0, bytes) != hipSuccess || hipMemset(b, 0, bytes) != hipSuccess) st = BF_ERR_DEVICE;
+    const dim3 grid((unsigned)Y, (unsigned)Z), block((unsigned)((X + 63) / 64 * 64));
+    int k = 0;
+    if (st == BF_OK)
+        st = time_launches([&](hipStream_t s) {
+            float *src = (k & 1) ? b : a, *dst = (k & 1) ? a : b;
+            hipLaunchKernelGGL(k_pull_pattern, grid, block, 0, s, src, dst, X, Y, Z);
+            ++k;
+        }, iters, &ms);
+    if (st == BF_OK) *gbps = 2.0 * sizeof(float) * 19.0 * (double)cells * iters / (ms * 1e-3) / 1e9;
+    (void)hipFree(a);
+    (void)hipFree(b);
     return st;
 }

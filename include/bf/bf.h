@@ -436,6 +436,14 @@ bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, int32_t slot
 bf_status bf_probe_aa_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t even, int32_t iters,
                                 double *gbps);
 
+/* Two-lattice pull probe (no counterpart in the reference), for comparison
+ * with the AA pattern: slot i of each cell is read from one D3Q19 lattice in
+ * the row (y - c_iy, z - c_iz) and written to a second lattice at the cell
+ * itself, at the cell's own x, on a periodic X x Y x Z grid in the slab layout,
+ * without arithmetic; launches alternate the two lattices.  X <= 1024.
+ * *gbps = read + write bytes per second over `iters` launches. */
+bf_status bf_probe_pull_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t iters, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

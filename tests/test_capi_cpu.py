@@ -135,3 +135,15 @@ def test_aa_probe_rejects_bad_arguments(bfa):
     assert lib.bf_probe_aa_bandwidth(0, 2048, 8, 8, 1, 1, C.byref(out)) == 1
     assert lib.bf_probe_aa_bandwidth(0, 0, 8, 8, 0, 1, C.byref(out)) == 1
     assert lib.bf_probe_aa_bandwidth(0, 64, 8, 8, 0, 1, None) == 1
+
+
+def test_pull_probe_rejects_bad_arguments(bfa):
+    """bf_probe_pull_bandwidth checks its arguments before any device call, as
+    the AA probe does."""
+    import ctypes as C
+    lib = bfa.load_library()
+    out = C.c_double()
+    assert lib.bf_probe_pull_bandwidth(0, 2048, 8, 8, 1, C.byref(out)) == 1
+    assert lib.bf_probe_pull_bandwidth(0, 8, 0, 8, 1, C.byref(out)) == 1
+    assert lib.bf_probe_pull_bandwidth(0, 64, 8, 8, 0, C.byref(out)) == 1
+    assert lib.bf_probe_pull_bandwidth(0, 64, 8, 8, 1, None) == 1
