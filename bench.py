@@ -28,6 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from boltzmannfluids_amd import probe  # noqa: E402  (bench-only bandwidth probes, libbfprobe.so)
 from boltzmannfluids_amd.scenarios import (lid_cavity, porosity, porous_spheres,  # noqa: E402
                                            spinodal, taylor_green)
 
@@ -295,7 +296,7 @@ def counters_sample(bfa, n=256, ticks=10):
     """Short workload for one rocprofv3 --pmc pass: a 1 GiB device copy of known
     traffic (counter calibration), then `ticks` ticks of the FP32 and FP16 runs."""
     import torch  # noqa: F401  (one HIP runtime with torch)
-    print("copy GB/s", round(bfa.capi.copy_bandwidth(1 << 30, 3), 1), flush=True)
+    print("copy GB/s", round(probe.copy_bandwidth(1 << 30, 3), 1), flush=True)
     rho, u = taylor_green(n)
     for st in ("fp32", "fp16"):
         s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19", storage=st)
@@ -438,18 +439,18 @@ def main():
     # achievable-bandwidth ceilings on this device: a plain copy, and the tick's
     # own access pattern (19 FP32 slots read and rewritten in place) without
     # arithmetic, at the FP32 lattice size and at the 16-bit lattice's bytes
-    copy = bfa.capi.copy_bandwidth(1 << 30, 20)
-    inplace = bfa.capi.inplace_bandwidth(n ** 3, 19, 20)
-    inplace_half = bfa.capi.inplace_bandwidth(n ** 3 // 2, 19, 20)
+    copy = probe.copy_bandwidth(1 << 30, 20)
+    inplace = probe.inplace_bandwidth(n ** 3, 19, 20)
+    inplace_half = probe.inplace_bandwidth(n ** 3 // 2, 19, 20)
     # the AA tick's two memory patterns (odd: every slot at the cell; even: the
     # y/z-shifted rows) at the workload's size; a tick pair takes 1/B_odd +
     # 1/B_even per byte, so their harmonic mean is the pair's ceiling
-    aa_odd = bfa.capi.aa_bandwidth(n, n, n, False, 20)
-    aa_even = bfa.capi.aa_bandwidth(n, n, n, True, 20)
+    aa_odd = probe.aa_bandwidth(n, n, n, False, 20)
+    aa_even = probe.aa_bandwidth(n, n, n, True, 20)
     aa_pair = 2.0 / (1.0 / aa_odd + 1.0 / aa_even)
     # the two-lattice pull pattern (y/z-shifted reads, local writes to a second
     # lattice) at the same size, for comparison with the AA tick pair
-    pull = bfa.capi.pull_bandwidth(n, n, n, 20)
+    pull = probe.pull_bandwidth(n, n, n, 20)
     line["ceiling"] = {"copy_GBs": round(copy, 1), "inplace_19slot_GBs": round(inplace, 1),
                        "inplace_19slot_half_bytes_GBs": round(inplace_half, 1),
                        "fp32_kernel_frac_of_inplace": round(main_run["roofline"]["achieved"] / inplace, 4),

@@ -1,4 +1,5 @@
-"""Build libbfamd.so (the HIP solver behind include/bf/bf.h) for gfx950.
+"""Build libbfamd.so (the HIP solver behind include/bf/bf.h) for gfx950, and
+libbfprobe.so (the benchmark's bandwidth probes, csrc/bf_probe.h).
 
 The step kernel source is compiled once per velocity set (-DBF_VS=<id>) so
 that the four sets build in parallel; the host runtime is one more object.
@@ -15,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbfamd.so")
+PROBE_LIB = os.path.join(HERE, "libbfprobe.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -48,17 +50,22 @@ def build(verbose=False, jobs=None, out=None, defines=()):
     for name, vs in VELOCITY_SETS.items():
         obj = os.path.join(build_dir, f"bf_step_{name}.o")
         units.append([HIPCC, *COMMON, *extra, f"-DBF_VS={vs}", "-c", os.path.join(CSRC, "bf_step.hip"), "-o", obj])
-    for src in ("bf_solver", "bf_probe", "bf_channel", "bf_mc", "bf_voxel"):
+    for src in ("bf_solver", "bf_channel", "bf_mc", "bf_voxel", "bf_probe"):
         units.append([HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src + ".hip"),
                       "-o", os.path.join(build_dir, src + ".o")])
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for r in ex.map(_run, units):
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
-    objs = [u[-1] for u in units]
+    objs = [u[-1] for u in units if not u[-1].endswith("bf_probe.o")]
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, lib)
+    if out is None:
+        # the benchmark's bandwidth probes: a library of their own
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", PROBE_LIB + ".tmp",
+              os.path.join(build_dir, "bf_probe.o")])
+        os.replace(PROBE_LIB + ".tmp", PROBE_LIB)
     return lib
 
 

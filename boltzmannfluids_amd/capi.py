@@ -110,10 +110,6 @@ def load_library(path=None):
         "bf_bits_per_boundary": (i32, [i32]),
         "bf_pack_boundaries": (i32, [C.POINTER(C.c_uint8), i64, i32, C.POINTER(C.c_uint32)]),
         "bf_unpack_boundaries": (i32, [C.POINTER(C.c_uint32), i64, i32, C.POINTER(C.c_uint8)]),
-        "bf_probe_copy_bandwidth": (i32, [i32, i64, i32, C.POINTER(C.c_double)]),
-        "bf_probe_inplace_bandwidth": (i32, [i32, i64, i32, i32, C.POINTER(C.c_double)]),
-        "bf_probe_aa_bandwidth": (i32, [i32, i32, i32, i32, i32, i32, C.POINTER(C.c_double)]),
-        "bf_probe_pull_bandwidth": (i32, [i32, i32, i32, i32, i32, C.POINTER(C.c_double)]),
         "bf_save_mesoscopic": (i32, [vp, C.c_char_p]),
         "bf_save_macroscopic": (i32, [vp, C.c_char_p]),
         "bf_load_mesoscopic": (i32, [vp, C.c_char_p, i64]),
@@ -197,55 +193,6 @@ def unpack_boundaries(words, n, bits):
     if st:
         raise BFError(st, "bf_unpack_boundaries")
     return ids
-
-
-def copy_bandwidth(nbytes=2 << 30, iters=20, device=0):
-    """Sustained device copy bandwidth in GB/s (read + write)."""
-    lib = load_library()
-    out = C.c_double()
-    st = lib.bf_probe_copy_bandwidth(int(device), int(nbytes), int(iters), C.byref(out))
-    if st:
-        raise BFError(st, "bf_probe_copy_bandwidth")
-    return out.value
-
-
-def inplace_bandwidth(cells, slots=19, iters=20, device=0):
-    """GB/s (read + write) of the tick's access pattern without arithmetic:
-    `slots` FP32 slots of `cells` cells in the lattice's slab layout, read and
-    rewritten in place."""
-    lib = load_library()
-    out = C.c_double()
-    st = lib.bf_probe_inplace_bandwidth(int(device), int(cells), int(slots), int(iters), C.byref(out))
-    if st:
-        raise BFError(st, "bf_probe_inplace_bandwidth")
-    return out.value
-
-
-def aa_bandwidth(X, Y, Z, even, iters=20, device=0):
-    """GB/s (read + write) of one D3Q19 AA tick's memory pattern on a periodic
-    X x Y x Z grid in the slab layout, without arithmetic: the odd tick
-    (even=False, every slot at the cell) or the even tick (even=True, slot i
-    read in row (y - c_iy, z - c_iz), slot opp(i) written in row (y + c_iy,
-    z + c_iz))."""
-    lib = load_library()
-    out = C.c_double()
-    st = lib.bf_probe_aa_bandwidth(int(device), int(X), int(Y), int(Z), int(bool(even)), int(iters), C.byref(out))
-    if st:
-        raise BFError(st, "bf_probe_aa_bandwidth")
-    return out.value
-
-
-def pull_bandwidth(X, Y, Z, iters=20, device=0):
-    """GB/s (read + write) of the two-lattice pull pattern for D3Q19 on a
-    periodic X x Y x Z grid in the slab layout, without arithmetic: slot i read
-    from one lattice in row (y - c_iy, z - c_iz), written to a second lattice
-    at the cell itself."""
-    lib = load_library()
-    out = C.c_double()
-    st = lib.bf_probe_pull_bandwidth(int(device), int(X), int(Y), int(Z), int(iters), C.byref(out))
-    if st:
-        raise BFError(st, "bf_probe_pull_bandwidth")
-    return out.value
 
 
 def marching_cubes_table():

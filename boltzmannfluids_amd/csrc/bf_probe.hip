@@ -1,18 +1,22 @@
-// bf_probe.hip — device bandwidth probes.
+// bf_probe.hip — device bandwidth probes (measurement infrastructure).
 //
-// bf_probe_copy_bandwidth streams `bytes` from one device buffer to another
+// Built into its own library, libbfprobe.so, which only the benchmark loads;
+// the solver library libbfamd.so does not contain these kernels.  Entry points
+// are declared in bf_probe.h and return the bf_status codes of bf.h.
+//
+// bfp_copy_bandwidth streams `bytes` from one device buffer to another
 // with 16-byte accesses and reports the sustained rate (read + write bytes per
 // second): the achievable HBM bandwidth of a plain copy, and a kernel of known
 // traffic for calibrating the FETCH_SIZE / WRITE_SIZE counters.
 //
-// bf_probe_inplace_bandwidth reproduces the memory traffic of one lattice tick
+// bfp_inplace_bandwidth reproduces the memory traffic of one lattice tick
 // without its arithmetic: every thread reads one 4-byte element of each of
 // `slots` slots of its cell, in the solver's slab layout (slabs of 2^kSlabLog
 // cells holding the slots one after the other), and writes a value back to the
 // same places.  Its rate is the ceiling of the in-place tick's access pattern.
 #include <hip/hip_runtime.h>
 
-#include "../../include/bf/bf.h"
+#include "bf_probe.h"
 #include "bf_kernels.h"
 #include "bf_lattice.h"
 
@@ -140,7 +144,7 @@ bf_status time_launches(F launch, int32_t iters, float *ms) {
 
 }  // namespace
 
-extern "C" bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t iters, double *gbps) {
+extern "C" bf_status bfp_copy_bandwidth(int32_t device, int64_t bytes, int32_t iters, double *gbps) {
     if (!gbps || bytes < 4096 || iters < 1) return BF_ERR_INVALID;
     if (hipSetDevice(device) != hipSuccess) return BF_ERR_DEVICE;
     const size_t n = (size_t)bytes / sizeof(float4);
@@ -162,7 +166,7 @@ extern "C" bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int3
     return st;
 }
 
-extern "C" bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, int32_t slots, int32_t iters,
+extern "C" bf_status bfp_inplace_bandwidth(int32_t device, int64_t cells, int32_t slots, int32_t iters,
                                                 double *gbps) {
     if (!gbps || cells < 64 || iters < 1) return BF_ERR_INVALID;
     if (!(slots == 5 || slots == 7 || slots == 9 || slots == 15 || slots == 19 || slots == kMaxSlots))
@@ -183,7 +187,7 @@ extern "C" bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, i
     return st;
 }
 
-extern "C" bf_status bf_probe_aa_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t even,
+extern "C" bf_status bfp_aa_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t even,
                                            int32_t iters, double *gbps) {
     if (!gbps || iters < 1 || X < 1 || X > 1024 || Y < 1 || Y > 65535 || Z < 1 || Z > 65535) return BF_ERR_INVALID;
     if (hipSetDevice(device) != hipSuccess) return BF_ERR_DEVICE;
@@ -206,7 +210,7 @@ extern "C" bf_status bf_probe_aa_bandwidth(int32_t device, int32_t X, int32_t Y,
     return st;
 }
 
-extern "C" bf_status bf_probe_pull_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t iters,
+extern "C" bf_status bfp_pull_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t iters,
                                              double *gbps) {
     if (!gbps || iters < 1 || X < 1 || X > 1024 || Y < 1 || Y > 65535 || Z < 1 || Z > 65535) return BF_ERR_INVALID;
     if (hipSetDevice(device) != hipSuccess) return BF_ERR_DEVICE;

@@ -957,14 +957,19 @@ bf_status bf_unpack_boundaries(const uint32_t *words, int64_t n, int32_t bits, u
 // the AA parity of the tick count; thermal populations; both pseudopotential buffers;
 // the force field; boundary ids; the saved macroscopic fields), so a loaded
 // solver continues bit for bit.  Macroscopic files hold the saved fields
-// (u, rho), (F, T) and the ids; loading one re-initialises the populations at
-// equilibrium like initialize_fields.
+// (u, rho), (F, T), the ids and the external force (the constant force in the
+// header, a per-cell force field as a section); loading one re-initialises the
+// populations at equilibrium like initialize_fields, from the saved (u, rho),
+// the saved T and the external force.  The F channel of the saved fields is
+// not a force to restore: it holds the total force of the save tick, with the
+// buoyancy and the Shan-Chen force added (collide.comp:707-720).
 
 namespace {
 
 constexpr char kMagic[8] = {'B', 'F', 'L', 'B', 'M', 'S', 'T', '1'};
-// version 2: the population section holds h = f - w for every storage type
-constexpr uint32_t kVersion = 2;
+// version 2: the population section holds h = f - w for every storage type;
+// version 3: macroscopic files carry the per-cell external force field
+constexpr uint32_t kVersion = 3;
 enum : uint32_t { kKindMacro = 0, kKindMeso = 1 };
 enum : uint32_t {
     kTagPopulations = 1, kTagThermal = 2, kTagPsi = 3, kTagForce = 4, kTagIds = 5, kTagVelocityDensity = 6,
@@ -1080,8 +1085,8 @@ bf_status write_state(bf_solver *s, const char *dir, uint32_t kind) {
             BF_HIP(s, hipMemcpy(ps.data() + N * sizeof(float), s->psi[1], N * sizeof(float), hipMemcpyDeviceToHost));
             ok = ok && put(file.f, kTagPsi, ps.data(), ps.size());
         }
-        if (s->forcing == 2 && (st = section(kTagForce, s->force_d, N * sizeof(float4))) != BF_OK) return st;
     }
+    if (s->forcing == 2 && (st = section(kTagForce, s->force_d, N * sizeof(float4))) != BF_OK) return st;
     if (s->has_ids && (st = section(kTagIds, s->ids, N)) != BF_OK) return st;
     if ((st = section(kTagVelocityDensity, s->vd, N * sizeof(float4))) != BF_OK) return st;
     if ((st = section(kTagForceTemperature, s->ft, N * sizeof(float4))) != BF_OK) return st;
@@ -1118,18 +1123,28 @@ bf_status bf_load_macroscopic(bf_solver *s, const char *directory, int64_t tick)
     bf_status st = open_state(s, directory, tick, kKindMacro, file, h);
     if (st != BF_OK) return st;
     const size_t N = (size_t)s->N;
+    auto truncated = [&]() { return fail(s, BF_ERR_INVALID, "[LBM Error] truncated checkpoint"); };
+    // the external force: the header's constant force, or the per-cell field
+    std::vector<float> field;
+    if (h.forcing == 2) {
+        field.resize(4 * N);
+        if (!get(file.f, kTagForce, field.data(), 16 * N)) return truncated();
+    }
     std::vector<uint8_t> ids;
     if (h.has_ids) {
         ids.resize(N);
-        if (!get(file.f, kTagIds, ids.data(), N)) return fail(s, BF_ERR_INVALID, "[LBM Error] truncated checkpoint");
+        if (!get(file.f, kTagIds, ids.data(), N)) return truncated();
     }
     std::vector<float> vd(4 * N), ft(4 * N);
     if (!get(file.f, kTagVelocityDensity, vd.data(), 16 * N) || !get(file.f, kTagForceTemperature, ft.data(), 16 * N))
-        return fail(s, BF_ERR_INVALID, "[LBM Error] truncated checkpoint");
+        return truncated();
     std::vector<float> rho(N), u(3 * N), force(3 * N), T(N);
     for (size_t c = 0; c < N; ++c) {
         rho[c] = vd[4 * c + 3];
-        for (int k = 0; k < 3; ++k) { u[3 * c + k] = vd[4 * c + k]; force[3 * c + k] = ft[4 * c + k]; }
+        for (int k = 0; k < 3; ++k) {
+            u[3 * c + k] = vd[4 * c + k];
+            force[3 * c + k] = h.forcing == 2 ? field[4 * c + k] : h.fconst[k];
+        }
         T[c] = ft[4 * c + 3];
     }
     apply_params(s, h);

@@ -340,8 +340,10 @@ bf_status bf_channel_timing(const bf_channel *c, double out[3]);
  * populations, pseudopotential buffers, force field, ids, parameters and the
  * object table): loading one into a solver of the same configuration continues
  * the run bit for bit.  Macroscopic files hold the saved fields (u, rho),
- * (F, T) and the ids: loading one re-initialises the populations at
- * equilibrium (bf_init_fields) and sets the tick count.  Loading checks that
+ * (F, T), the ids and the external force (constant or per-cell field):
+ * loading one re-initialises the populations at equilibrium (bf_init_fields
+ * with the saved u, rho and T and the external force, not the saved F, which
+ * includes buoyancy and the Shan-Chen force) and sets the tick count.  Loading checks that
  * the file's grid, velocity set, multiphase flag, boundary mode, periodicity
  * (and storage, for mesoscopic files) match the solver. */
 bf_status bf_save_mesoscopic(bf_solver *solver, const char *directory);
@@ -410,39 +412,6 @@ bf_status bf_voxelize_mesh(int32_t device, const float *vertices, int64_t vertex
                            uint8_t *volume, int64_t *solid_count);
 
 const char *bf_voxelize_last_error(void);
-
-/* --- diagnostics (no counterpart in the reference) ---------------------- */
-
-/* Device copy bandwidth: copies `bytes` between two device buffers `iters`
- * times and returns (read + write) GB/s.  The achievable-HBM figure beside the
- * published 8 TB/s, and the known-traffic kernel for counter calibration. */
-bf_status bf_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t iters, double *gbps);
-
-/* In-place streaming probe (no counterpart in the reference): the memory
- * traffic of one lattice tick without its arithmetic — every cell reads and
- * rewrites one FP32 element in each of `slots` (5, 7, 9, 15, 19 or 27)
- * slots of `cells` cells in the lattice's slab layout.  *gbps = read + write bytes
- * per second over `iters` launches: the ceiling of the tick's access pattern. */
-bf_status bf_probe_inplace_bandwidth(int32_t device, int64_t cells, int32_t slots, int32_t iters, double *gbps);
-
-/* AA-pattern probe (no counterpart in the reference): the memory traffic of
- * one D3Q19 tick of the in-place AA update on a periodic X x Y x Z grid in the
- * lattice's slab layout, without arithmetic.  even = 0: every cell reads and
- * rewrites its 19 slots at the cell itself (the odd tick); even = 1: slot i
- * is read in the row (y - c_iy, z - c_iz) and slot opp(i) written in the row
- * (y + c_iy, z + c_iz), at the cell's own x (the even tick, whose x shift goes
- * through LDS).  X <= 1024.  *gbps = read + write bytes per second over
- * `iters` launches: the ceiling of each tick's access pattern. */
-bf_status bf_probe_aa_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t even, int32_t iters,
-                                double *gbps);
-
-/* Two-lattice pull probe (no counterpart in the reference), for comparison
- * with the AA pattern: slot i of each cell is read from one D3Q19 lattice in
- * the row (y - c_iy, z - c_iz) and written to a second lattice at the cell
- * itself, at the cell's own x, on a periodic X x Y x Z grid in the slab layout,
- * without arithmetic; launches alternate the two lattices.  X <= 1024.
- * *gbps = read + write bytes per second over `iters` launches. */
-bf_status bf_probe_pull_bandwidth(int32_t device, int32_t X, int32_t Y, int32_t Z, int32_t iters, double *gbps);
 
 #ifdef __cplusplus
 }

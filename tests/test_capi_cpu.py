@@ -70,10 +70,13 @@ def test_rest_population_mass_fix(orc, name, thermal):
         assert total != 1 and fix < 0 and np.log2(-float(fix)) == int(np.log2(-float(fix)))
 
 
-def test_bits_per_boundary_matches_oracle(bfa, orc):
-    lib = orc.load()
-    for n in range(0, 257):
-        assert bfa.bits_per_boundary(n) == lib.orc_bits_per_boundary(n), n
+def test_bits_per_boundary_known_answers(bfa):
+    """The library's rule against the known answers of LBM.cpp:569 (worked by
+    hand per range of counts in test_oracle.expected_bits), not against the
+    oracle's restatement of the same formula."""
+    from test_oracle import expected_bits
+    for n in range(0, 300):
+        assert bfa.bits_per_boundary(n) == expected_bits(n), n
 
 
 @pytest.mark.parametrize("bits", [1, 2, 4, 8])
@@ -125,25 +128,33 @@ def test_field_copy_rejects_null_handles(bfa):
     assert lib.bf_copy_field_to_device(None, 0, None, 16) == 1
 
 
-def test_aa_probe_rejects_bad_arguments(bfa):
-    """bf_probe_aa_bandwidth checks its arguments before any device call: rows
+def test_probes_live_outside_the_solver_library(bfa):
+    """The bandwidth probes are the benchmark's (libbfprobe.so); the solver
+    library exports none of them."""
+    lib = bfa.load_library()
+    for name in ("bf_probe_copy_bandwidth", "bf_probe_aa_bandwidth", "bfp_copy_bandwidth", "bfp_aa_bandwidth"):
+        assert not hasattr(lib, name), name
+
+
+def test_aa_probe_rejects_bad_arguments():
+    """bfp_aa_bandwidth checks its arguments before any device call: rows
     longer than one block (X > 1024), empty grids and a null result are
     BF_ERR_INVALID (1)."""
-    import ctypes as C
-    lib = bfa.load_library()
+    from boltzmannfluids_amd import probe
+    lib = probe.load_library()
     out = C.c_double()
-    assert lib.bf_probe_aa_bandwidth(0, 2048, 8, 8, 1, 1, C.byref(out)) == 1
-    assert lib.bf_probe_aa_bandwidth(0, 0, 8, 8, 0, 1, C.byref(out)) == 1
-    assert lib.bf_probe_aa_bandwidth(0, 64, 8, 8, 0, 1, None) == 1
+    assert lib.bfp_aa_bandwidth(0, 2048, 8, 8, 1, 1, C.byref(out)) == 1
+    assert lib.bfp_aa_bandwidth(0, 0, 8, 8, 0, 1, C.byref(out)) == 1
+    assert lib.bfp_aa_bandwidth(0, 64, 8, 8, 0, 1, None) == 1
 
 
-def test_pull_probe_rejects_bad_arguments(bfa):
-    """bf_probe_pull_bandwidth checks its arguments before any device call, as
-    the AA probe does."""
-    import ctypes as C
-    lib = bfa.load_library()
+def test_pull_probe_rejects_bad_arguments():
+    """bfp_pull_bandwidth checks its arguments before any device call, as the
+    AA probe does."""
+    from boltzmannfluids_amd import probe
+    lib = probe.load_library()
     out = C.c_double()
-    assert lib.bf_probe_pull_bandwidth(0, 2048, 8, 8, 1, C.byref(out)) == 1
-    assert lib.bf_probe_pull_bandwidth(0, 8, 0, 8, 1, C.byref(out)) == 1
-    assert lib.bf_probe_pull_bandwidth(0, 64, 8, 8, 0, C.byref(out)) == 1
-    assert lib.bf_probe_pull_bandwidth(0, 64, 8, 8, 1, None) == 1
+    assert lib.bfp_pull_bandwidth(0, 2048, 8, 8, 1, C.byref(out)) == 1
+    assert lib.bfp_pull_bandwidth(0, 8, 0, 8, 1, C.byref(out)) == 1
+    assert lib.bfp_pull_bandwidth(0, 64, 8, 8, 0, C.byref(out)) == 1
+    assert lib.bfp_pull_bandwidth(0, 64, 8, 8, 1, None) == 1

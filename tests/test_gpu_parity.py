@@ -124,26 +124,65 @@ def test_aa_addressing_all_periodicities(bfa, orc, vs, periodic):
     assert np.array_equal(g.read("velocity_density").astype(np.float64), o.read("velocity_density"))
 
 
+def assert_16bit_parity(a, b, storage, vs, orc, min_identical=0.98):
+    """16-bit storage against the oracle's emulated storage: identical except
+    for one-quantum rounding flips (at least `min_identical` of the values
+    identical; every difference within two quanta of the value's own
+    magnitude, quantum 2^-11 (FP16) or 2^-8 (BF16) relative to |h| = |f - w|)."""
+    d = np.abs(a - b)
+    q = 2.0 ** -11 if storage == "fp16" else 2.0 ** -8
+    w = orc.velocity_table(vs)[:, 3][:, None]
+    same = float(np.mean(a == b))
+    say(f"  16-bit: identical={same:.5f}, max |diff| / quantum = "
+        f"{(d / (q * np.maximum(np.abs(b - w), 2.0 ** -14))).max():.2f}")
+    assert same >= min_identical
+    assert np.all(d <= 2 * q * np.maximum(np.abs(b - w), 2.0 ** -14) + 1e-9)
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("mode", ["halfway", "fullway_compat"])
 @pytest.mark.parametrize("vs,periodic", [("D2Q9", (px, py, 1)) for px in (0, 1) for py in (0, 1)] +
                          [("D3Q19", p) for p in PERIODIC_COMBOS_3D])
-def test_walls_all_periodicities(bfa, orc, vs, periodic, mode):
+def test_walls_all_periodicities(bfa, orc, vs, periodic, mode, storage):
     """Random solid cells (15 %, two objects, one a moving wall) on a 7 x 5 x 3
     grid (7 x 5 for D2Q9) with every combination of periodic and open axes,
-    both wall modes: 5 ticks, bit-identical to the oracle (bounce-back and
-    the moving-wall term are the same FP32 operations in both)."""
+    both wall modes and every storage type: 5 ticks.  FP32 bit-identical to
+    the oracle (bounce-back and the moving-wall term are the same FP32
+    operations in both); FP16/BF16 (the wall kernels of the porous workload's
+    16-bit runs) against the oracle's emulated storage at quantum level."""
     res = (7, 5, 1) if vs == "D2Q9" else (7, 5, 3)
     n = int(np.prod(res))
     rng = np.random.default_rng(SEED + 29)
     ids = np.where(rng.random(n) < 0.15, rng.integers(1, 3, n), 0).astype(np.uint8)
     rho, u = random_state(n, seed=SEED + 29, du=0.06, drho=0.04)
     objs = [(2, dict(velocity=(0.02, -0.01, 0.0)))]
-    g, o = run_pair(bfa, orc, res, 0.6, 5, velocity_set=vs, periodic=periodic, boundary_mode=mode, objects=objs,
-                    init=dict(rho=rho, u=u, boundary_id=ids))
+    g, o = run_pair(bfa, orc, res, 0.6, 5, velocity_set=vs, storage=storage, periodic=periodic, boundary_mode=mode,
+                    objects=objs, init=dict(rho=rho, u=u, boundary_id=ids))
     a, b = g.read("populations_raw").astype(np.float64), o.read("populations_raw")
     say(f"  raw max|diff|={np.abs(a - b).max():.3e} identical={np.mean(a == b):.4f}")
-    assert np.array_equal(a, b)
-    assert np.array_equal(g.read("velocity_density").astype(np.float64), o.read("velocity_density"))
+    if storage == "fp32":
+        assert np.array_equal(a, b)
+        assert np.array_equal(g.read("velocity_density").astype(np.float64), o.read("velocity_density"))
+    else:
+        assert_16bit_parity(a, b, storage, vs, orc)
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+@pytest.mark.parametrize("mode", ["halfway", "fullway_compat"])
+def test_walls_16bit_rows_of_whole_waves(bfa, orc, storage, mode):
+    """The 16-bit wall kernels on rows that fill whole waves (X = 64, 128):
+    random solids, a moving wall, periodic x and open y/z edges, 12 ticks
+    ending on an even tick, against the oracle's emulated storage."""
+    for res in ((64, 6, 5), (128, 4, 3)):
+        n = int(np.prod(res))
+        rng = np.random.default_rng(SEED + 31)
+        ids = np.where(rng.random(n) < 0.1, rng.integers(1, 3, n), 0).astype(np.uint8)
+        rho, u = random_state(n, seed=SEED + 31, du=0.06, drho=0.04)
+        objs = [(2, dict(velocity=(0.02, -0.01, 0.01)))]
+        g, o = run_pair(bfa, orc, res, 0.6, 12, velocity_set="D3Q19", storage=storage, periodic=(1, 0, 0),
+                        boundary_mode=mode, objects=objs, init=dict(rho=rho, u=u, boundary_id=ids))
+        assert_16bit_parity(g.read("populations_raw").astype(np.float64), o.read("populations_raw"), storage,
+                            "D3Q19", orc)
 
 
 @pytest.mark.parametrize("res,vs,periodic,storage", [
@@ -237,11 +276,16 @@ def test_constant_force_porous_duct(bfa, orc):
     ids = spheres_ids(X, Y, Z, 0.05, 2, 4, seed=SEED + 4)
     force = np.tile(np.array([1e-6, 0, 0], np.float32), (n, 1))
     init = dict(rho=np.ones(n, np.float32), u=np.zeros((n, 3), np.float32), force=force, boundary_id=ids)
-    for st in ("fp32", "fp16"):
+    for st in ("fp32", "fp16", "bf16"):
         g, o = run_pair(bfa, orc, (X, Y, Z), 0.56, 60, storage=st, periodic=(1, 0, 0), init=init)
         assert g.info()["forcing"] == 1
         if st == "fp32":
             assert_fp32_parity(g, o, atol=1e-6)
+        else:
+            # the kernel of the porous workload's 16-bit runs (walls, constant
+            # force) against the oracle's emulated 16-bit storage
+            assert_16bit_parity(g.read("populations_raw").astype(np.float64), o.read("populations_raw"), st,
+                                "D3Q19", orc)
 
 
 def test_shan_chen_d3q27(bfa, orc):

@@ -290,17 +290,29 @@ def test_thermal_conduction_linear_profile(orc):
     np.testing.assert_allclose(T[1:Y - 1], exact, rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize("count", list(range(0, 258)))
-def test_bits_per_boundary_rule(orc, count):
-    """LBM.cpp:569: bits = 2^ceil(log2(ceil(log2(count)))), at most 8, and
-    enough to hold ids 0..count-1."""
-    b = orc.load().orc_bits_per_boundary(count)
+def expected_bits(count):
+    """Known answers of LBM.cpp:569, bits = exp2(ceil(log2(ceil(log2(count))))),
+    worked by hand per range of counts (count includes the fluid slot, id 0):
+    1 object -> no boundary field; 2 -> 1 bit; 3-4 -> 2; 5-16 -> 4; 17-256 -> 8;
+    257 and more -> 16, which the reference rejects (LBM.cpp:570-573)."""
     if count <= 1:
-        assert b == 0
-        return
-    assert b in (1, 2, 4, 8) or count > 256
+        return 0
+    if count == 2:
+        return 1
+    if count <= 4:
+        return 2
+    if count <= 16:
+        return 4
     if count <= 256:
-        assert (1 << b) >= count
+        return 8
+    return 16
+
+
+@pytest.mark.parametrize("count", list(range(0, 300)))
+def test_bits_per_boundary_rule(orc, count):
+    """The exact value of the reference's rule for every count up to 299 (not
+    just a width that holds the ids: an "always 8" rule would fail)."""
+    assert orc.load().orc_bits_per_boundary(count) == expected_bits(count)
 
 
 @pytest.mark.parametrize("bits", [1, 2, 4, 8])
