@@ -44,6 +44,7 @@ struct StepArgs {
     uint32_t slab_log, slab_skip, pair_step;
     float *gslot[kMaxQT];    // base of thermal slot q (g + q * stride)
     int X, Y, Z;
+    int z0;                  // first plane of the launch (grid.z planes from z0)
     int periodic_x, periodic_y, periodic_z;
     const uint8_t *ids;      // boundary id per cell (or null)
     const uint32_t *nmask;   // solid-neighbour mask per cell (see launch_mask), with ids
@@ -118,6 +119,15 @@ int x_direction_count(int velocity_set);
 template <int VS>
 hipError_t launch_step(const StepArgs &a, int storage, bool odd, bool walls, bool multiphase,
                        bool thermal, const LaunchShape &shape, double *partial, hipStream_t stream);
+
+// One launch of a chunked tick pair (non-multiphase kernels): the even tick
+// on `nze` planes from ae.z0 and the odd tick on `nzo` planes from ao.z0, in
+// one grid.  The two plane ranges must not overlap, and the even range must
+// not reach within one plane of the odd range (see k_pair).  `partial`
+// receives the odd tick's FP64 block sums when non-null.
+template <int VS>
+hipError_t launch_pair(const StepArgs &ae, const StepArgs &ao, int nze, int nzo, int storage, bool walls, bool thermal,
+                       const LaunchShape &shape, double *partial, hipStream_t stream);
 
 // Neighbour mask of every cell: bit 0 = the cell is solid, bit i (1..Q-1) =
 // the neighbour at x + c_i lies inside the domain (periodic axes wrap) and is

@@ -62,6 +62,7 @@ struct bf_solver {
     float tau = 0.53f, G = -6.0f, beta = 0.5f, tau_t = 2.0f, t_ref = 1.0f, cs2_t = 1.0f / 3.0f;
     float ulim = 0.25f, uext = 0.30f;
     int sc_sync = 0;  // synchronous Shan-Chen pseudopotential (density pre-pass per tick)
+    int pair_planes = 0;  // chunked tick pairs: 0 automatic, -1 off, k > 0 chunks of k planes
     float fconst[3] = {0, 0, 0};
     int forcing = 0, thermal = 0, has_ids = 0, initialized = 0;
     int cur = 0;
@@ -639,6 +640,11 @@ bf_status bf_set_param(bf_solver *s, int32_t key, const float *v, int32_t n) {
     case BF_PARAM_VELOCITY_LIMIT: s->ulim = v[0]; return BF_OK;
     case BF_PARAM_VELOCITY_LIMIT_EXTREME: s->uext = v[0]; return BF_OK;
     case BF_PARAM_SHAN_CHEN_SYNCHRONOUS: s->sc_sync = v[0] != 0.0f; return BF_OK;
+    case BF_PARAM_TICK_PAIR_PLANES:
+        if (!(v[0] >= -1.0f) || v[0] != std::floor(v[0]) || v[0] > 65535.0f)
+            return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: tick-pair chunk must be -1, 0 or a plane count");
+        s->pair_planes = (int)v[0];
+        return BF_OK;
     }
     return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: unknown key %d", key);
 }
@@ -658,17 +664,112 @@ bf_status bf_get_param(const bf_solver *s, int32_t key, float *v, int32_t n) {
     case BF_PARAM_VELOCITY_LIMIT: v[0] = s->ulim; return BF_OK;
     case BF_PARAM_VELOCITY_LIMIT_EXTREME: v[0] = s->uext; return BF_OK;
     case BF_PARAM_SHAN_CHEN_SYNCHRONOUS: v[0] = (float)s->sc_sync; return BF_OK;
+    case BF_PARAM_TICK_PAIR_PLANES: v[0] = (float)s->pair_planes; return BF_OK;
     }
     return BF_ERR_INVALID;
 }
 
+namespace {
+
+// Chunked tick pairs.  An even tick followed by an odd tick can run chunk by
+// chunk of z-planes: the odd tick of a chunk needs the even tick of that chunk
+// and of its two neighbour chunks (the even tick writes into the planes next
+// to its own), so chunk j-2's odd tick can share a launch with chunk j's even
+// tick (run_pair).  The lines the even tick wrote are then still in the
+// 256 MB last-level cache when the odd tick reads them.  Measured on the
+// MI355X (D3Q19, device time per tick): the 384^3 porous duct 1.76 -> 1.59 ms
+// (FP32, chunks of 7 planes) and 1.08 -> 1.05 ms (FP16, 14 planes); the
+// fully periodic 256^3 box gains nothing (FP32 411 -> 419-434 us, FP16 202 ->
+// 224-273 us, chunks of 8-32 planes), although the same schedule of its
+// access pattern alone moves 7.47 instead of 6.35 TB/s.  So the automatic
+// mode pairs the ticks of grids with walls whose lattice exceeds the cache
+// (kPairMinBytes), in chunks of about kPairChunkBytes.
+constexpr double kPairChunkBytes = 80e6;
+constexpr double kPairMinBytes = 384e6;
+
+// planes per chunk, or 0 for whole-grid ticks
+int pair_chunk(const bf_solver *s) {
+    if (s->cfg.multiphase || s->pair_planes < 0) return 0;
+    int zc = s->pair_planes;
+    if (zc == 0) {
+        if (!s->has_ids) return 0;
+        const double plane = (double)s->X * s->Y * ((double)s->q * s->elem + (s->thermal ? 4.0 * s->qt : 0.0));
+        if (plane * s->Z < kPairMinBytes) return 0;
+        zc = std::max(1, (int)std::lround(kPairChunkBytes / plane));
+    }
+    // the schedule needs at least four chunks (chunk 0's odd tick waits for
+    // the last chunk's even tick, which writes into plane 0 on a periodic z axis)
+    return (s->Z + zc - 1) / zc >= 4 ? zc : 0;
+}
+
+hipError_t dispatch_pair(const bf_solver *s, const StepArgs &ae, const StepArgs &ao, int nze, int nzo,
+                         double *partial) {
+    hipStream_t stream = s->stream;
+    const bool walls = s->has_ids != 0, th = s->thermal != 0;
+    switch (s->vs) {
+    case D2Q9: return launch_pair<D2Q9>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
+    case D3Q15: return launch_pair<D3Q15>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
+    case D3Q19: return launch_pair<D3Q19>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
+    case D3Q27: return launch_pair<D3Q27>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+// An even tick and the odd tick after it, in chunks of zc planes (m >= 4
+// chunks), on the handle's stream:
+//   E0 | E1 | E2 | E3 + O1 | ... | E(m-1) + O(m-3) | O(m-2) + O(m-1) | O0
+// The odd tick of chunk l needs the even ticks of chunks l-1, l and l+1 (the
+// even tick writes into the planes beside its own; chunks 0 and m-1 are
+// neighbours on a periodic z axis), all of them in earlier launches; within a
+// launch the even chunk j and the odd chunk j-2 are at least one plane apart.
+// `flags` apply to the odd tick.
+bf_status run_pair(bf_solver *s, int zc, uint32_t flags) {
+    const int Z = s->Z, m = (Z + zc - 1) / zc;
+    StepArgs ae = make_args(s, true), ao = make_args(s, true);
+    if (flags & BF_STEP_SAVE_MACRO) { ao.vd = s->vd; ao.ft = s->ft; }
+    double *partial = (flags & BF_STEP_CONSERVATION) ? s->partial : nullptr;
+    auto lo = [&](int j) { return j * zc; };
+    auto hi = [&](int j) { return std::min(Z, (j + 1) * zc); };
+    // even tick on chunk je (none if je < 0), odd tick on chunks [jo0, jo1)
+    auto launch = [&](int je, int jo0, int jo1) -> bf_status {
+        int nze = 0, nzo = 0;
+        if (je >= 0) { ae.z0 = lo(je); nze = hi(je) - lo(je); }
+        if (jo0 < jo1) { ao.z0 = lo(jo0); nzo = hi(jo1 - 1) - lo(jo0); }
+        const hipError_t e = dispatch_pair(s, ae, ao, nze, nzo, partial);
+        if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] tick-pair launch failed: %s", hipGetErrorString(e));
+        return BF_OK;
+    };
+    bf_status st = BF_OK;
+    for (int j = 0; j < m && st == BF_OK; ++j) st = j >= 3 ? launch(j, j - 2, j - 1) : launch(j, 0, 0);
+    if (st == BF_OK) st = launch(-1, m - 2, m);
+    if (st == BF_OK) st = launch(-1, 0, 1);
+    if (st != BF_OK) return st;
+    if (partial) {
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] reduction launch failed: %s", hipGetErrorString(e));
+    }
+    s->ticks += 2;
+    return BF_OK;
+}
+
+}  // namespace
+
 static bf_status run_ticks(bf_solver *s, int64_t n, uint32_t flags) {
+    const int zc = pair_chunk(s);
     for (int64_t k = 0; k < n; ++k) {
         const bool last = k == n - 1;
+        const bool odd = ((s->ticks - s->tick_base) & 1) != 0;
+        if (zc > 0 && !odd && k + 1 < n) {
+            // an even tick and the odd tick after it, chunk by chunk
+            const bf_status st = run_pair(s, zc, k + 2 == n ? flags : 0u);
+            if (st != BF_OK) return st;
+            ++k;
+            continue;
+        }
         StepArgs a = make_args(s, true);
         if (last && (flags & BF_STEP_SAVE_MACRO)) { a.vd = s->vd; a.ft = s->ft; }
         double *partial = (last && (flags & BF_STEP_CONSERVATION)) ? s->partial : nullptr;
-        const bool odd = ((s->ticks - s->tick_base) & 1) != 0;
         if (s->cfg.multiphase && s->sc_sync) {
             // density pre-pass: the same reads without writes of populations;
             // psi of this tick's density goes to psi[cur], which the tick then

@@ -48,21 +48,18 @@ __device__ inline uint16_t bf16_rne(float v) {
 #define BF_NT_STORE 1
 #endif
 
-template <class T>
+// NT selects the streaming form; the even tick of a chunked tick pair uses
+// ordinary accesses instead, so that the lines it writes stay in the last-level
+// cache for the odd tick of the same chunk (see k_pair).
+template <bool NT, class T>
 __device__ inline T ld(const T *p) {
-#if BF_NT_LOAD
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
+    if constexpr (NT && BF_NT_LOAD) return __builtin_nontemporal_load(p);
+    else return *p;
 }
-template <class T>
+template <bool NT, class T>
 __device__ inline void st(T *p, T v) {
-#if BF_NT_STORE
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
+    if constexpr (NT && BF_NT_STORE) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 // Element `idx` of a slot whose base is wave-uniform, addressed as base + a
@@ -123,36 +120,36 @@ __device__ inline const T *at_bytes(const T *p, uint32_t off) {
 // the solver accepts, so their byte offset is 32-bit (scalar-base form); FP32
 // pairs use a 64-bit address (an FP32 lattice too large for slabs keeps plain
 // slot arrays, whose pairs span more than 4 GiB).
-template <int ST> struct Store;
-template <> struct Store<ST_FP32> {
+template <int ST, bool NTL, bool NTS> struct Store;
+template <bool NTL, bool NTS> struct Store<ST_FP32, NTL, NTS> {
     using T = float;
-    __device__ static T raw_pair(const T *p, uint32_t e) { return ld(p + (size_t)e); }
-    __device__ static void store_pair(T *p, uint32_t e, float h) { st(p + (size_t)e, h); }
-    __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
+    __device__ static T raw_pair(const T *p, uint32_t e) { return ld<NTL>(p + (size_t)e); }
+    __device__ static void store_pair(T *p, uint32_t e, float h) { st<NTS>(p + (size_t)e, h); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld<NTL>(at(p, idx)); }
     __device__ static float cvt(T v) { return v; }
-    __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), h); }
-    __device__ static T raw_b(const T *p, uint32_t off) { return ld(at_bytes(p, off)); }
-    __device__ static void store_b(T *p, uint32_t off, float h) { st(at_bytes(p, off), h); }
+    __device__ static void store(T *p, uint32_t idx, float h) { st<NTS>(at(p, idx), h); }
+    __device__ static T raw_b(const T *p, uint32_t off) { return ld<NTL>(at_bytes(p, off)); }
+    __device__ static void store_b(T *p, uint32_t off, float h) { st<NTS>(at_bytes(p, off), h); }
 };
-template <> struct Store<ST_FP16> {
+template <bool NTL, bool NTS> struct Store<ST_FP16, NTL, NTS> {
     using T = _Float16;
-    __device__ static T raw_pair(const T *p, uint32_t e) { return ld(at(p, e)); }
-    __device__ static void store_pair(T *p, uint32_t e, float h) { st(at(p, e), (_Float16)h); }
-    __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
+    __device__ static T raw_pair(const T *p, uint32_t e) { return ld<NTL>(at(p, e)); }
+    __device__ static void store_pair(T *p, uint32_t e, float h) { st<NTS>(at(p, e), (_Float16)h); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld<NTL>(at(p, idx)); }
     __device__ static float cvt(T v) { return (float)v; }
-    __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), (_Float16)h); }
-    __device__ static T raw_b(const T *p, uint32_t off) { return ld(at_bytes(p, off)); }
-    __device__ static void store_b(T *p, uint32_t off, float h) { st(at_bytes(p, off), (_Float16)h); }
+    __device__ static void store(T *p, uint32_t idx, float h) { st<NTS>(at(p, idx), (_Float16)h); }
+    __device__ static T raw_b(const T *p, uint32_t off) { return ld<NTL>(at_bytes(p, off)); }
+    __device__ static void store_b(T *p, uint32_t off, float h) { st<NTS>(at_bytes(p, off), (_Float16)h); }
 };
-template <> struct Store<ST_BF16> {
+template <bool NTL, bool NTS> struct Store<ST_BF16, NTL, NTS> {
     using T = uint16_t;
-    __device__ static T raw_pair(const T *p, uint32_t e) { return ld(at(p, e)); }
-    __device__ static void store_pair(T *p, uint32_t e, float h) { st(at(p, e), bf16_rne(h)); }
-    __device__ static T raw(const T *p, uint32_t idx) { return ld(at(p, idx)); }
+    __device__ static T raw_pair(const T *p, uint32_t e) { return ld<NTL>(at(p, e)); }
+    __device__ static void store_pair(T *p, uint32_t e, float h) { st<NTS>(at(p, e), bf16_rne(h)); }
+    __device__ static T raw(const T *p, uint32_t idx) { return ld<NTL>(at(p, idx)); }
     __device__ static float cvt(T v) { return __uint_as_float((uint32_t)v << 16); }
-    __device__ static void store(T *p, uint32_t idx, float h) { st(at(p, idx), bf16_rne(h)); }
-    __device__ static T raw_b(const T *p, uint32_t off) { return ld(at_bytes(p, off)); }
-    __device__ static void store_b(T *p, uint32_t off, float h) { st(at_bytes(p, off), bf16_rne(h)); }
+    __device__ static void store(T *p, uint32_t idx, float h) { st<NTS>(at(p, idx), bf16_rne(h)); }
+    __device__ static T raw_b(const T *p, uint32_t off) { return ld<NTL>(at_bytes(p, off)); }
+    __device__ static void store_b(T *p, uint32_t off, float h) { st<NTS>(at_bytes(p, off), bf16_rne(h)); }
 };
 
 // c_i . v summed over the non-zero components of c_i in x, y, z order.  Equal
@@ -208,8 +205,11 @@ __device__ inline int wrap_index(int v, int n) {
     return (unsigned)v < (unsigned)n ? v : ((v % n) + n) % n;
 }
 
-template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER>
-__global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
+// One tick of the cells of plane z covered by this block (the body of k_step
+// and of both halves of k_pair).  NTL / NTS: streaming (nontemporal)
+// population loads / stores.
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, bool NTL, bool NTS>
+__device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict__ partial, const int z) {
     // PER: every axis periodic (and no walls): every neighbour exists, so the
     // open-boundary branches vanish at compile time.
     // CPT cells per thread along x, blockDim.x apart: all their loads are issued
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     static_assert(!(PER && (WALLS || MP || TH)), "the periodic specialisation is the plain kernel");
     extern __shared__ float lds_rows[];
     using L = Lattice<VS>;
-    using S = Store<ST>;
+    using S = Store<ST, NTL, NTS>;
     using T = typename S::T;
     constexpr int Q = L::Q;
     constexpr int QT = L::QT;
@@ -246,7 +246,6 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
     // load in range and store nothing), so no access needs a per-lane guard
     const int y_in = blockIdx.y * blockDim.y + ty;
     const int y = PER ? min(y_in, Y - 1) : y_in;
-    const int z = blockIdx.z;
 
     // Neighbour coordinates: index 0 = -1, 1 = 0, 2 = +1 along each axis.
     // Written as selects: a branch here would split the block that issues the
@@ -786,10 +785,51 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
             const int nw = (blockDim.x * blockDim.y + 63) >> 6;
             double acc = 0.0;
             for (int wv = 0; wv < nw; ++wv) acc += red[wv][tid];
-            const size_t b = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+            // block index within the whole grid (a launch may cover some planes only)
+            const size_t b = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * (size_t)z);
             partial[b * 5 + tid] = acc;
         }
     }
+}
+
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER>
+__global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
+    step_body<VS, ST, ODD, WALLS, MP, TH, CPT, AL, PER, true, true>(a, partial, a.z0 + (int)blockIdx.z);
+}
+
+// A launch of a chunked tick pair: the blocks of the first `nze` planes run
+// the even tick on planes [ae.z0, ae.z0 + nze), the others the odd tick on
+// planes from ao.z0.  The host schedules chunk j's even tick with chunk j-2's
+// odd tick, so the two halves touch disjoint planes (the even tick of a plane
+// reads and writes its neighbour planes z +- 1; the odd tick stays at its
+// cells).  The even tick's accesses are ordinary loads and stores, whose
+// lines the 256 MB last-level cache keeps; the odd tick of that chunk, two
+// launches later, finds most of them there and streams its own traffic with
+// nontemporal hints.  (Which hints the two halves use measured within noise
+// of each other on the tick kernels; they decide the rate of the bare access
+// pattern.)
+#ifndef BF_PAIR_EVEN_NT_LOAD
+#define BF_PAIR_EVEN_NT_LOAD 0
+#endif
+#ifndef BF_PAIR_EVEN_NT_STORE
+#define BF_PAIR_EVEN_NT_STORE 0
+#endif
+#ifndef BF_PAIR_ODD_NT_LOAD
+#define BF_PAIR_ODD_NT_LOAD 1
+#endif
+#ifndef BF_PAIR_ODD_NT_STORE
+#define BF_PAIR_ODD_NT_STORE 1
+#endif
+#define BF_OCCUPANCY_PAIR __attribute__((amdgpu_waves_per_eu(TH ? 1 : BF_WAVES_PER_EU)))
+template <int VS, int ST, bool WALLS, bool TH, int CPT, bool AL, bool PER>
+__global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs ae, const StepArgs ao, const int nze,
+                                                                double *__restrict__ partial) {
+    if ((int)blockIdx.z < nze)
+        step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
+            ae, nullptr, ae.z0 + (int)blockIdx.z);
+    else
+        step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
+            ao, partial, ao.z0 + (int)blockIdx.z - nze);
 }
 
 // Equilibrium initialisation (set_equilibrium_populations.comp:198-220; the
@@ -799,7 +839,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 template <int VS, int ST, bool TH>
 __global__ void __launch_bounds__(256) k_init(const InitArgs a) {
     using L = Lattice<VS>;
-    using S = Store<ST>;
+    using S = Store<ST, true, true>;
     using T = typename S::T;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n) return;
@@ -904,6 +944,36 @@ hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const
 }
 
 template <int ST>
+hipError_t launch_pair_st(const StepArgs &ae, const StepArgs &ao, int nze, int nzo, bool walls, bool th,
+                          const LaunchShape &sh, double *partial, hipStream_t stream) {
+    constexpr int VS = BF_VS;
+    constexpr int CPT = cells_per_thread(ST, false, false);
+    // as for single ticks: the row buffer for the plain kernels and FP32 walls
+    const bool al = sh.lds_bytes > 0 && !(walls && ST != ST_FP32);
+    const dim3 grid(sh.grid.x, sh.grid.y, (unsigned)(nze + nzo));
+    const size_t lds = al ? sh.lds_bytes : 0;
+#define BF_P(W, T, A, P)                                                                                   \
+    do {                                                                                                   \
+        hipLaunchKernelGGL((k_pair<VS, ST, W, T, T ? 1 : CPT, A, P>), grid, sh.block, lds, stream, ae, ao, nze, \
+                           partial);                                                                       \
+        return hipGetLastError();                                                                          \
+    } while (0)
+    if (!walls && !th && ae.periodic_x && ae.periodic_y && ae.periodic_z) {
+        if (al) BF_P(false, false, true, true);
+        BF_P(false, false, false, true);
+    }
+    if (walls) {
+        if (th) { if (al) BF_P(true, true, true, false); BF_P(true, true, false, false); }
+        if (al) BF_P(true, false, true, false);
+        BF_P(true, false, false, false);
+    }
+    if (th) { if (al) BF_P(false, true, true, false); BF_P(false, true, false, false); }
+    if (al) BF_P(false, false, true, false);
+    BF_P(false, false, false, false);
+#undef BF_P
+}
+
+template <int ST>
 hipError_t launch_init_st(const InitArgs &a, bool th, hipStream_t stream) {
     const dim3 block(256), grid((unsigned)((a.n + 255) / 256));
     if (th) hipLaunchKernelGGL((k_init<BF_VS, ST, true>), grid, block, 0, stream, a);
@@ -923,6 +993,17 @@ hipError_t launch_step<BF_VS>(const StepArgs &a, int storage, bool odd, bool wal
                              : launch_step_st<ST_FP16, false>(a, walls, mp, th, sh, partial, stream);
     case ST_BF16: return odd ? launch_step_st<ST_BF16, true>(a, walls, mp, th, sh, partial, stream)
                              : launch_step_st<ST_BF16, false>(a, walls, mp, th, sh, partial, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <>
+hipError_t launch_pair<BF_VS>(const StepArgs &ae, const StepArgs &ao, int nze, int nzo, int storage, bool walls,
+                              bool th, const LaunchShape &sh, double *partial, hipStream_t stream) {
+    switch (storage) {
+    case ST_FP32: return launch_pair_st<ST_FP32>(ae, ao, nze, nzo, walls, th, sh, partial, stream);
+    case ST_FP16: return launch_pair_st<ST_FP16>(ae, ao, nze, nzo, walls, th, sh, partial, stream);
+    case ST_BF16: return launch_pair_st<ST_BF16>(ae, ao, nze, nzo, walls, th, sh, partial, stream);
     }
     return hipErrorInvalidValue;
 }

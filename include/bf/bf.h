@@ -107,7 +107,18 @@ typedef enum bf_param {
      * pre-pass per tick gives every neighbour's psi of the current tick
      * (costs one extra read of the lattice per tick).  No counterpart in the
      * reference. */
-    BF_PARAM_SHAN_CHEN_SYNCHRONOUS = 9
+    BF_PARAM_SHAN_CHEN_SYNCHRONOUS = 9,
+    /* 1 float (default 0): how bf_step runs an even tick and the odd tick
+     * after it.  -1 = each tick over the whole grid; k > 0 = the pair chunk
+     * by chunk of k z-planes, chunk j's even tick in one launch with chunk
+     * j-2's odd tick, so that the odd tick reads what the even tick wrote
+     * from the last-level cache; 0 = chunks of about 80 MB of lattice for
+     * grids with walls whose lattice exceeds 384 MB (where it was measured
+     * faster), else whole-grid ticks.  Results are
+     * identical in every mode (each cell sees the same operations); the
+     * Shan-Chen solver always runs whole-grid ticks, and so does a grid of
+     * fewer than four chunks.  No counterpart in the reference. */
+    BF_PARAM_TICK_PAIR_PLANES = 10
 } bf_param;
 
 /* Fields for bf_read_field. */

@@ -1,0 +1,153 @@
+"""Chunked tick pairs (BF_PARAM_TICK_PAIR_PLANES): an even tick and the odd tick
+after it run z-chunk by z-chunk, chunk j's even tick in one launch with chunk
+j-2's odd tick.  Every cell sees the same operations as with whole-grid ticks,
+so the results must be bit-identical to the whole-grid ticks and to the oracle
+(FP32), or identical to the oracle's emulated 16-bit storage up to one-quantum
+rounding flips, whatever the chunk size: chunks of 1, 2 and 3 planes, a ragged
+last chunk, both parities at the start and end of a bf_step call, saves and
+FP64 sums on the last tick, walls, open z faces and the thermal lattice."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import SEED, channel_ids, random_state, spheres_ids
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_pair(bfa, orc, res, vs, storage, planes, *, periodic=(1, 1, 1), walls=False, thermal=False, seed=SEED + 51):
+    X, Y, Z = res
+    n = X * Y * Z
+    rho, u = random_state(n, seed=seed, du=0.06, drho=0.04)
+    init = dict(rho=rho, u=u)
+    objects, params = [], []
+    if walls:
+        rng = np.random.default_rng(seed)
+        ids = np.where(rng.random(n) < 0.1, rng.integers(1, 3, n), 0).astype(np.uint8)
+        init["boundary_id"] = ids
+        objects.append((2, dict(velocity=(0.02, 0.0, -0.01))))
+    if thermal:
+        ids = channel_ids(X, Y, Z)
+        ids.reshape(Z, Y, X)[:, -1, :] = 2
+        init["boundary_id"] = ids
+        init["temperature"] = (0.5 + 0.02 * np.random.default_rng(seed).random(n)).astype(np.float32)
+        init["force"] = np.tile(np.array([0, -1e-4, 0], np.float32), (n, 1))
+        objects = [(1, dict(temperature=1.0)), (2, dict(temperature=0.0))]
+        params = [("reference_temperature", 0.5), ("thermal_relaxation_time", 0.9)]
+    g = bfa.Solver(res, 0.6, periodic=periodic, velocity_set=vs, storage=storage)
+    o = orc.Oracle(res, 0.6, periodic=periodic, velocity_set=vs, storage=storage, precision="f32")
+    g.set_param("tick_pair_planes", planes)
+    for oid, kw in objects:
+        g.set_object(oid, **kw)
+        o.set_object(oid, **kw)
+    for k, v in params:
+        g.set_param(k, v)
+        o.set_param(k, v)
+    g.init_fields(**init)
+    o.init_fields(**init)
+    return g, o
+
+
+def check(g, o, storage, vs, orc, thermal=False):
+    a, b = g.read("populations_raw").astype(np.float64), o.read("populations_raw")
+    same = float(np.mean(a == b))
+    say(f"  raw max|diff|={np.abs(a - b).max():.3e} identical={same:.5f}")
+    if storage == "fp32":
+        assert np.array_equal(a, b)
+        assert np.array_equal(g.read("velocity_density").astype(np.float64), o.read("velocity_density"))
+    else:
+        q = 2.0 ** -11 if storage == "fp16" else 2.0 ** -8
+        w = orc.velocity_table(vs)[:, 3][:, None]
+        assert same > 0.98
+        assert np.all(np.abs(a - b) <= 2 * q * np.maximum(np.abs(b - w), 2.0 ** -14) + 1e-9)
+    if thermal:
+        assert np.array_equal(g.read("thermal_raw").astype(np.float64), o.read("thermal_raw"))
+
+
+CASES = [("D3Q19", "fp32", 1, (12, 10, 8)), ("D3Q19", "fp32", 3, (12, 10, 13)), ("D3Q19", "fp16", 2, (16, 6, 9)),
+         ("D3Q19", "bf16", 2, (16, 6, 9)), ("D3Q15", "fp32", 2, (10, 8, 8)), ("D3Q27", "fp32", 2, (10, 8, 11)),
+         ("D3Q27", "fp16", 3, (10, 8, 12)), ("D3Q19", "fp32", 2, (64, 5, 9)), ("D3Q19", "fp16", 2, (128, 3, 8))]
+
+
+@pytest.mark.parametrize("vs,storage,planes,res", CASES)
+def test_pairs_periodic_match_oracle(bfa, orc, vs, storage, planes, res):
+    """Fully periodic grids (the plain kernels, with and without the row
+    buffer): 2 + 3 + 4 ticks in three calls, so pairs start on both parities
+    and calls end on both; saved fields on the last tick of each call."""
+    say(f"case: pairs {vs} {storage} {res} planes={planes}")
+    g, o = make_pair(bfa, orc, res, vs, storage, planes)
+    for n in (2, 3, 4):
+        g.step(n, save=True)
+        o.step(n, save=True)
+        check(g, o, storage, vs, orc)
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+@pytest.mark.parametrize("periodic", [(1, 0, 1), (1, 0, 0), (0, 1, 0)])
+def test_pairs_walls_and_open_faces_match_oracle(bfa, orc, storage, periodic):
+    """Random solid cells with a moving wall, open y or z faces (0.01 reads
+    and dropped writes at z = 0 and z = Z-1 cross chunk boundaries)."""
+    res = (14, 9, 11)
+    say(f"case: pairs walls {storage} periodic={periodic}")
+    g, o = make_pair(bfa, orc, res, "D3Q19", storage, 2, periodic=periodic, walls=True)
+    for n in (4, 5):
+        g.step(n, save=True)
+        o.step(n, save=True)
+        check(g, o, storage, "D3Q19", orc)
+
+
+@pytest.mark.parametrize("vs", ["D3Q19", "D3Q27"])
+def test_pairs_thermal_match_oracle(bfa, orc, vs):
+    """The thermal lattice (D3Q7, in place like the main lattice) in pairs:
+    hot and cold walls with buoyancy."""
+    say(f"case: pairs thermal {vs}")
+    g, o = make_pair(bfa, orc, (12, 10, 10), vs, "fp32", 2, periodic=(1, 0, 1), thermal=True)
+    assert g.info()["thermal"] == 1
+    for n in (6, 3):
+        g.step(n, save=True)
+        o.step(n, save=True)
+        check(g, o, "fp32", vs, orc, thermal=True)
+
+
+def test_pairs_conservation_sums(bfa, orc):
+    """FP64 sums recorded on the last tick of a pair: each odd chunk writes
+    the partials of its own planes."""
+    g, o = make_pair(bfa, orc, (12, 10, 9), "D3Q19", "fp32", 2, periodic=(1, 0, 1), walls=True)
+    g.step(6, conservation=True)
+    o.step(6, conservation=True)
+    np.testing.assert_allclose(g.last_step_sums()[:4], o.last_step_sums()[:4], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_pairs_equal_whole_grid_ticks_porous_96(bfa, storage):
+    """A 96^3 porous duct (walls, constant force) run with chunks of 4 planes
+    and with whole-grid ticks: bit-identical after 10 ticks."""
+    n = 96
+    cells = n ** 3
+    ids = spheres_ids(n, n, n, 0.05, 4, 8, seed=SEED + 53)
+    out = []
+    for planes in (-1, 4):
+        s = bfa.Solver((n, n, n), 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", storage=storage)
+        s.set_param("tick_pair_planes", planes)
+        s.init_fields(np.ones(cells, np.float32), np.zeros((cells, 3), np.float32),
+                      force=np.tile(np.array([1e-5, 0, 0], np.float32), (cells, 1)), boundary_id=ids)
+        s.step(10, save=True)
+        out.append((s.read("populations_raw"), s.read("velocity_density")))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_pairs_parameter(bfa):
+    s = bfa.Solver((8, 8, 8), 0.6)
+    assert s.get_param("tick_pair_planes") == 0
+    s.set_param("tick_pair_planes", 3)
+    assert s.get_param("tick_pair_planes") == 3
+    for bad in (-2, 1.5):
+        with pytest.raises(bfa.BFError):
+            s.set_param("tick_pair_planes", bad)
