@@ -10,10 +10,14 @@ Prints one JSON line:
   ceiling                 copy and in-place 19-slot streaming probes (achievable GB/s)
   fp16_shifted, bf16_...  the same run with 16-bit shifted storage
   error_vs_fp32           16-bit fields vs the FP32 run (same workload at 128^3)
+  error_vs_fp64           every storage type vs the FP64 CPU oracle (TGV and porous duct
+                          at 64^3, 1000 ticks; computed in the CPU leg)
   other_configs           porous 384^3 D3Q19 (FP32/FP16/BF16, + error at 128^3),
                           Shan-Chen D3Q27 + thermal D3Q7 192^3, D2Q9 cavity 256^2
   adjoint_over_forward    differentiable channel: gradient time / forward time
   cpu_baseline            the FP32 CPU restatement (oracle) on the host cores
+  wall_clock_frac         the headline's HBM fraction from host wall clock (roofline.frac
+                          uses the kernels' device time)
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -268,16 +272,34 @@ def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
         n *= r
     state_bytes = 4 * (19 if vs == "D3Q19" else 9) * ((n + 63) // 64 * 64)
     resident = min(steps // k, max(1, budget // (state_bytes * k))) if budget else 0
-    return {"workload": f"channel {vs} {'x'.join(map(str, res))}, tau 0.6, u_in 0.05, {steps} ticks, "
-                        f"checkpoint every {k}",
+    return {"workload": f"channel {vs} {'x'.join(map(str, res))}, tau 0.6, u_in 0.05, porosity 0.02 x box-smoothed "
+                        f"U[0,1], {steps} ticks, checkpoint every {k}",
             "trajectory_budget_GB": round(budget / 1e9, 1), "resident_segments": resident,
             "segments": steps // k,
             "J": J, "forward_ms": round(fwd, 3), "gradient_ms": round(t[2], 3), "recompute_ms": round(t[1], 3),
             "ratio": round(t[2] / fwd, 3), "dJ_duin": du}, dg
 
 
-def cpu_baseline(n=128, ticks=400):
-    """FP32 CPU restatement (oracle/liboracle_f32.so) on the host cores, bounded sample."""
+def _gpu_fields(bfa, shape, tau, storage, ticks, setup, periodic=(1, 1, 1)):
+    g = bfa.Solver(shape, tau, periodic=periodic, velocity_set="D3Q19", storage=storage)
+    setup(g)
+    g.step(ticks, save=True)
+    vd = g.read("velocity_density").astype(np.float64)
+    g.close()
+    return vd
+
+
+def cpu_leg(bfa, storages, n_err=64, ticks_err=1000, n=128, ticks=400):
+    """The benchmark's CPU leg, the only part of it that runs the CPU oracle
+    (oracle/, test infrastructure):
+      cpu_baseline  the FP32 restatement (oracle/liboracle_f32.so) timed on the
+                    host cores on a bounded sample of the headline workload;
+      error_vs_fp64 the FP64 oracle run on the Taylor-Green box and on the
+                    porous duct at n_err^3 for ticks_err ticks, as the absolute
+                    reference for the GPU runs of every storage type on the
+                    same inputs: ||d rho||/||rho|| and ||d u||/||u|| over the
+                    fluid cells (the oracle is the checker here, never the
+                    thing measured)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     rho, u = taylor_green(n)
@@ -287,9 +309,50 @@ def cpu_baseline(n=128, ticks=400):
     t0 = time.perf_counter()
     o.step(ticks)
     dt = time.perf_counter() - t0
-    return {"value": round(n ** 3 * ticks / dt / 1e6, 3), "unit": "MLUPs/s", "cores": threads, "kind": "port",
+    o.close()
+    base = {"value": round(n ** 3 * ticks / dt / 1e6, 3), "unit": "MLUPs/s", "cores": threads, "kind": "port",
             "sample": f"D3Q19 TGV {n}^3 FP32, {ticks} ticks, oracle/liboracle_f32.so, {threads} OpenMP threads "
                       f"({dt:.1f} s)"}
+    if not storages:
+        return base, None
+    m = n_err
+    cells = m ** 3
+    errors = {"workloads": f"D3Q19 {m}^3, {ticks_err} ticks, against the FP64 oracle (fluid cells)"}
+    t0 = time.perf_counter()
+    # Taylor-Green box (tau 0.55, u0 0.05, fully periodic)
+    rho, u = taylor_green(m)
+    o = oracle.Oracle((m, m, m), 0.55, velocity_set="D3Q19", precision="f64")
+    o.init_fields(rho, u)
+    o.step(ticks_err, save=True)
+    ref = o.read("velocity_density")
+    o.close()
+    errors["tgv"] = {}
+    for st in storages:
+        vd = _gpu_fields(bfa, (m, m, m), 0.55, st, ticks_err, lambda g: g.init_fields(rho, u))
+        er, eu = field_error(vd, ref)
+        errors["tgv"][st] = {"rho": er, "u": eu}
+    # porous duct (periodic x, walls y/z, spheres r 4-12, g_x 1e-6, tau 0.56)
+    ids = porous_spheres(m, m, m)
+    fluid = ids == 0
+    force = np.array([1e-6, 0, 0], np.float32)
+
+    def setup(s):
+        # (initialisation without a force array resets the force; the constant force follows it)
+        s.init_fields(np.ones(cells, np.float32), np.zeros((cells, 3), np.float32), boundary_id=ids)
+        s.set_param("constant_force", force)
+
+    o = oracle.Oracle((m, m, m), 0.56, periodic=(1, 0, 0), velocity_set="D3Q19", precision="f64")
+    setup(o)
+    o.step(ticks_err, save=True)
+    ref = o.read("velocity_density")[fluid]
+    o.close()
+    errors["porous"] = {"solid_fraction": round(float(1 - fluid.mean()), 4)}
+    for st in storages:
+        vd = _gpu_fields(bfa, (m, m, m), 0.56, st, ticks_err, setup, periodic=(1, 0, 0))[fluid]
+        er, eu = field_error(vd, ref)
+        errors["porous"][st] = {"rho": er, "u": eu}
+    errors["oracle_seconds"] = round(time.perf_counter() - t0, 1)
+    return base, errors
 
 
 def counters_sample(bfa, n=256, ticks=10):
@@ -435,6 +498,10 @@ def main():
         "config": {"workload": f"D3Q19 BGK esoteric-pull, {n}^3 fully periodic, TGV u0=0.05, tau=0.55, FP32 storage",
                    "cells": n ** 3, "bytes_per_cell": bytes_per_cell(19, "fp32")},
         "roofline": main_run["roofline"],
+        # roofline.frac divides the algorithmic bytes by the kernels' device
+        # time (HIP events on the solver's stream); the same bytes over the
+        # host wall clock of the timed region (value, ms_per_step):
+        "wall_clock_frac": round(bytes_per_cell(19, "fp32") * main_run["mlups"] * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
     }
     # achievable-bandwidth ceilings on this device: a plain copy, and the tick's
     # own access pattern (19 FP32 slots read and rewritten in place) without
@@ -483,7 +550,9 @@ def main():
             runs += [r0, r1]
         line["adjoint_over_forward"] = runs
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"], err64 = cpu_leg(bfa, [] if args.no_accuracy else stores)
+        if err64:
+            line["error_vs_fp64"] = err64
     print(json.dumps(line), flush=True)
 
 
