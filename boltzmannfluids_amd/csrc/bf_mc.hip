@@ -401,7 +401,90 @@ struct DevBuf {
                            "[MarchingCubes Error] %s failed: %s", #call, hipGetErrorString(e_));       \
     } while (0)
 
-// The iso-surface of a device field `v` (value of point p at v[p * step]).
+// Device buffers of one extraction.  The per-point and per-cube counts are
+// scanned in place into first indices (k_scan_apply reads each element before
+// the same thread writes it).
+struct Work {
+    uint8_t *flags = nullptr, *cases = nullptr;     // np, nc
+    uint32_t *vfirst = nullptr, *tfirst = nullptr;  // np, nc
+    uint32_t *sums = nullptr;                       // 2 (nbmax + 1)
+    int64_t np = 0, nc = 0, nbmax = 0;
+};
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr int64_t scan_blocks_max(int64_t np) { return (np + kScanItems - 1) / kScanItems + 1; }
+// bytes of the work buffers for np points (nc <= np cubes), each part 256-byte aligned
+constexpr size_t work_bytes(int64_t np) {
+    return 2 * align256((size_t)np) + 2 * align256(4 * (size_t)np) + align256(4 * 2 * (size_t)(scan_blocks_max(np) + 1));
+}
+
+// carve the work buffers for np points and nc cubes out of `base`
+Work carve(uint8_t *base, int64_t np, int64_t nc) {
+    Work w;
+    w.np = np;
+    w.nc = nc;
+    w.nbmax = scan_blocks_max(std::max(np, nc));
+    size_t off = 0;
+    auto take = [&](size_t b) { uint8_t *p = base + off; off += align256(b); return p; };
+    w.flags = take((size_t)np);
+    w.cases = take((size_t)nc);
+    w.vfirst = reinterpret_cast<uint32_t *>(take(4 * (size_t)np));
+    w.tfirst = reinterpret_cast<uint32_t *>(take(4 * (size_t)nc));
+    w.sums = reinterpret_cast<uint32_t *>(take(4 * 2 * (size_t)(w.nbmax + 1)));
+    return w;
+}
+
+// Classification and prefix sums of the field g; *nv, *nt = vertex and
+// triangle counts (synchronises s to read them).
+bf_status count_pass(const Vol &g, const Work &w, uint32_t *nv, uint32_t *nt, hipStream_t s) {
+    const McTable &t = table();
+    MC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ntri), t.ntri, sizeof t.ntri, 0, hipMemcpyHostToDevice, s));
+    MC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_tri), t.tri, sizeof t.tri, 0, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mc_points, dim3((unsigned)((w.np + 255) / 256)), dim3(256), 0, s, g, w.flags, w.vfirst);
+    MC_HIP(hipGetLastError());
+    MC_HIP(scan(w.vfirst, w.np, w.vfirst, w.sums, s));
+    const int64_t nbp = (w.np + kScanItems - 1) / kScanItems;
+    uint32_t *sums_c = w.sums + w.nbmax + 1;
+    int64_t nbc = 0;
+    if (w.nc > 0) {
+        hipLaunchKernelGGL(k_mc_cubes, dim3((unsigned)((w.nc + 255) / 256)), dim3(256), 0, s, g, w.cases, w.tfirst);
+        MC_HIP(hipGetLastError());
+        MC_HIP(scan(w.tfirst, w.nc, w.tfirst, sums_c, s));
+        nbc = (w.nc + kScanItems - 1) / kScanItems;
+    }
+    *nv = 0;
+    *nt = 0;
+    MC_HIP(hipMemcpyAsync(nv, w.sums + nbp, sizeof *nv, hipMemcpyDeviceToHost, s));
+    if (w.nc > 0) MC_HIP(hipMemcpyAsync(nt, sums_c + nbc, sizeof *nt, hipMemcpyDeviceToHost, s));
+    MC_HIP(hipStreamSynchronize(s));
+    return BF_OK;
+}
+
+// Emit nv vertices (3 floats each) into vout and nt triangles (3 uint32 each)
+// into iout, both device memory.
+bf_status emit_pass(const Vol &g, const Work &w, uint32_t nv, uint32_t nt, const float mesh_scale[3], float *vout,
+                    uint32_t *iout, hipStream_t s) {
+    if (nv) {
+        hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)((w.np + 255) / 256)), dim3(256), 0, s, g, w.flags, w.vfirst,
+                           mesh_scale[0], mesh_scale[1], mesh_scale[2], vout);
+        MC_HIP(hipGetLastError());
+    }
+    if (nt) {
+        hipLaunchKernelGGL(k_mc_triangles, dim3((unsigned)((w.nc + 255) / 256)), dim3(256), 0, s, g, w.cases, w.tfirst,
+                           w.flags, w.vfirst, iout);
+        MC_HIP(hipGetLastError());
+    }
+    return BF_OK;
+}
+
+bf_status capacity_error(uint32_t nv, uint32_t nt, int64_t vertex_capacity, int64_t index_capacity) {
+    return mc_fail(BF_ERR_INVALID,
+                   "[MarchingCubes Error] output too small: %lld vertices and %lld indices needed, capacity %lld / %lld",
+                   (long long)nv, 3LL * nt, (long long)vertex_capacity, (long long)index_capacity);
+}
+
+// The iso-surface of a device field `v` (value of point p at v[p * step]) into
+// host arrays, with device buffers of its own.
 bf_status extract(const float *v, int step, const int32_t res[3], float threshold, const float mesh_scale[3],
                   float *vertices, int64_t vertex_capacity, uint32_t *indices, int64_t index_capacity,
                   int64_t *vertex_count, int64_t *index_count, hipStream_t s) {
@@ -410,41 +493,19 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
     const int64_t nc = (int64_t)std::max(X - 1, 0) * std::max(Y - 1, 0) * std::max(Z - 1, 0);
     *vertex_count = 0;
     *index_count = 0;
-    const McTable &t = table();
-    MC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ntri), t.ntri, sizeof t.ntri, 0, hipMemcpyHostToDevice, s));
-    MC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_tri), t.tri, sizeof t.tri, 0, hipMemcpyHostToDevice, s));
     DevBuf mem;
-    uint8_t *flags = nullptr, *cases = nullptr;
-    uint32_t *vcount = nullptr, *vfirst = nullptr, *tcount = nullptr, *tfirst = nullptr, *sums = nullptr;
-    const int64_t nbmax = (std::max(np, nc) + kScanItems - 1) / kScanItems + 1;
-    MC_HIP(mem.alloc(&flags, (size_t)np));
-    MC_HIP(mem.alloc(&vcount, sizeof(uint32_t) * (size_t)np));
-    MC_HIP(mem.alloc(&vfirst, sizeof(uint32_t) * (size_t)np));
-    MC_HIP(mem.alloc(&cases, (size_t)nc));
-    MC_HIP(mem.alloc(&tcount, sizeof(uint32_t) * (size_t)nc));
-    MC_HIP(mem.alloc(&tfirst, sizeof(uint32_t) * (size_t)nc));
-    MC_HIP(mem.alloc(&sums, sizeof(uint32_t) * (size_t)(2 * nbmax + 2)));
+    uint8_t *base = nullptr;
+    MC_HIP(mem.alloc(&base, work_bytes(np)));
+    const Work w = carve(base, np, nc);
     const Vol g{v, step, X, Y, Z, threshold};
     Events ev;
     g_mc_device_ms = 0.0;
     MC_HIP(hipEventRecord(ev.e[0], s));
-    hipLaunchKernelGGL(k_mc_points, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, flags, vcount);
-    MC_HIP(hipGetLastError());
-    MC_HIP(scan(vcount, np, vfirst, sums, s));
-    const int64_t nbp = (np + kScanItems - 1) / kScanItems;
-    uint32_t *sums_c = sums + nbmax + 1;
-    int64_t nbc = 0;
-    if (nc > 0) {
-        hipLaunchKernelGGL(k_mc_cubes, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, g, cases, tcount);
-        MC_HIP(hipGetLastError());
-        MC_HIP(scan(tcount, nc, tfirst, sums_c, s));
-        nbc = (nc + kScanItems - 1) / kScanItems;
-    }
-    MC_HIP(hipEventRecord(ev.e[1], s));
     uint32_t nv = 0, nt = 0;
-    MC_HIP(hipMemcpyAsync(&nv, sums + nbp, sizeof nv, hipMemcpyDeviceToHost, s));
-    if (nc > 0) MC_HIP(hipMemcpyAsync(&nt, sums_c + nbc, sizeof nt, hipMemcpyDeviceToHost, s));
-    MC_HIP(hipStreamSynchronize(s));
+    bf_status st = count_pass(g, w, &nv, &nt, s);
+    if (st != BF_OK) return st;
+    MC_HIP(hipEventRecord(ev.e[1], s));
+    MC_HIP(hipEventSynchronize(ev.e[1]));
     *vertex_count = nv;
     *index_count = 3 * (int64_t)nt;
     float ms = 0.0f;
@@ -452,30 +513,51 @@ bf_status extract(const float *v, int step, const int32_t res[3], float threshol
     g_mc_device_ms = ms;
     if (!vertices && !indices) return BF_OK;  // size query
     if ((nv && (!vertices || vertex_capacity < (int64_t)nv)) || (nt && (!indices || index_capacity < 3 * (int64_t)nt)))
-        return mc_fail(BF_ERR_INVALID,
-                       "[MarchingCubes Error] output too small: %lld vertices and %lld indices needed, capacity %lld / %lld",
-                       (long long)nv, 3LL * nt, (long long)vertex_capacity, (long long)index_capacity);
+        return capacity_error(nv, nt, vertex_capacity, index_capacity);
     float *vout = nullptr;
     uint32_t *iout = nullptr;
     MC_HIP(mem.alloc(&vout, sizeof(float) * 3 * (size_t)nv));
     MC_HIP(mem.alloc(&iout, sizeof(uint32_t) * 3 * (size_t)nt));
     MC_HIP(hipEventRecord(ev.e[2], s));
-    if (nv) {
-        hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, flags, vfirst,
-                           mesh_scale[0], mesh_scale[1], mesh_scale[2], vout);
-        MC_HIP(hipGetLastError());
-    }
-    if (nt) {
-        hipLaunchKernelGGL(k_mc_triangles, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, g, cases, tfirst,
-                           flags, vfirst, iout);
-        MC_HIP(hipGetLastError());
-    }
+    if ((st = emit_pass(g, w, nv, nt, mesh_scale, vout, iout, s)) != BF_OK) return st;
     MC_HIP(hipEventRecord(ev.e[3], s));
     if (nv) MC_HIP(hipMemcpyAsync(vertices, vout, sizeof(float) * 3 * (size_t)nv, hipMemcpyDeviceToHost, s));
     if (nt) MC_HIP(hipMemcpyAsync(indices, iout, sizeof(uint32_t) * 3 * (size_t)nt, hipMemcpyDeviceToHost, s));
     MC_HIP(hipStreamSynchronize(s));
     MC_HIP(hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
     g_mc_device_ms += ms;
+    return BF_OK;
+}
+
+// ---- allocation-free extraction -------------------------------------------
+//
+// bf_marching_cubes_noalloc carves its work buffers out of `scratch`, a
+// file-scope device array of work_bytes(BF_MC_NOALLOC_MAX_POINTS) bytes (about
+// 40 MiB, held by every GPU the library runs on until the process ends).  On
+// the host, `scratch` names a stand-in, not device memory: its device address
+// on each GPU comes from hipGetSymbolAddress, once per device with that device
+// current, and only that address is passed to kernels.  Every call on a GPU
+// shares its array, so calls that overlap on one GPU race; every buffer the
+// passes read is written earlier in the same call (nothing needs clearing).
+constexpr size_t kNoAllocScratchBytes = work_bytes(BF_MC_NOALLOC_MAX_POINTS);
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on `device` (the current device)
+bf_status scratch_on(int device, uint8_t **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return mc_fail(BF_ERR_DEVICE, "[MarchingCubes Error] hipGetSymbolAddress(scratch) failed: %s",
+                           hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<uint8_t *>(p);
+    }
+    *out = g_scratch_ptr[device];
     return BF_OK;
 }
 
@@ -535,6 +617,45 @@ bf_status bf_marching_cubes(int32_t device, const float *volume, const int32_t r
     if (dv) (void)hipFree(dv);
     (void)hipStreamDestroy(s);
     return st;
+}
+
+bf_status bf_marching_cubes_noalloc(int32_t device, const float *volume, const int32_t resolution[3], float threshold,
+                                    const float mesh_scale[3], float *vertices, int64_t vertex_capacity,
+                                    uint32_t *indices, int64_t index_capacity, int64_t *vertex_count,
+                                    int64_t *index_count, void *stream) {
+    if (!volume || !resolution || !mesh_scale || !vertex_count || !index_count)
+        return mc_fail(BF_ERR_INVALID, "[MarchingCubes Error] null argument");
+    if (resolution[0] <= 0 || resolution[1] <= 0 || resolution[2] <= 0)
+        return mc_fail(BF_ERR_INVALID, "[MarchingCubes Error] resolution must be positive");
+    const int64_t np = (int64_t)resolution[0] * resolution[1] * resolution[2];
+    if (np > BF_MC_NOALLOC_MAX_POINTS)
+        return mc_fail(BF_ERR_INVALID,
+                       "[MarchingCubes Error] %lld points exceed the %d points of the allocation-free extraction",
+                       (long long)np, BF_MC_NOALLOC_MAX_POINTS);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count || device >= kMaxDevices)
+        return mc_fail(BF_ERR_DEVICE, "[MarchingCubes Error] no device %d", device);
+    if (hipSetDevice(device) != hipSuccess) return mc_fail(BF_ERR_DEVICE, "[MarchingCubes Error] no device %d", device);
+    uint8_t *base = nullptr;
+    bf_status st = scratch_on(device, &base);
+    if (st != BF_OK) return st;
+    const int X = resolution[0], Y = resolution[1], Z = resolution[2];
+    const int64_t nc = (int64_t)std::max(X - 1, 0) * std::max(Y - 1, 0) * std::max(Z - 1, 0);
+    const Work w = carve(base, np, nc);
+    const Vol g{volume, 1, X, Y, Z, threshold};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    *vertex_count = 0;
+    *index_count = 0;
+    uint32_t nv = 0, nt = 0;
+    if ((st = count_pass(g, w, &nv, &nt, s)) != BF_OK) return st;
+    *vertex_count = nv;
+    *index_count = 3 * (int64_t)nt;
+    if (!vertices && !indices) return BF_OK;  // size query
+    if ((nv && (!vertices || vertex_capacity < (int64_t)nv)) || (nt && (!indices || index_capacity < 3 * (int64_t)nt)))
+        return capacity_error(nv, nt, vertex_capacity, index_capacity);
+    if ((st = emit_pass(g, w, nv, nt, mesh_scale, vertices, indices, s)) != BF_OK) return st;
+    MC_HIP(hipStreamSynchronize(s));
+    return BF_OK;
 }
 
 bf_status bf_isosurface(bf_solver *solver, float threshold, const float mesh_scale[3], float *vertices,

@@ -392,6 +392,23 @@ bf_status bf_isosurface(bf_solver *solver, float threshold, const float mesh_sca
                         int64_t vertex_capacity, uint32_t *indices, int64_t index_capacity, int64_t *vertex_count,
                         int64_t *index_count);
 
+/* Allocation-free variant of bf_marching_cubes, for callers that cannot
+ * allocate device memory (tight serving or rendering loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  `volume` (X*Y*Z
+ * floats), `vertices` and `indices` are device memory on `device`; `stream`
+ * is a hipStream_t (NULL: the default stream), on which the passes run and
+ * which is synchronised before return.  Same output, counts and size query
+ * (NULL outputs) as bf_marching_cubes.  The work buffers (point and cube
+ * classifications, prefix sums) come from a fixed device array of about
+ * 40 MiB that the library holds on each GPU it runs on: at most
+ * BF_MC_NOALLOC_MAX_POINTS grid points (BF_ERR_INVALID beyond), and calls
+ * that overlap on one GPU race.  Calls no hipMalloc or hipFree. */
+#define BF_MC_NOALLOC_MAX_POINTS 4194304
+bf_status bf_marching_cubes_noalloc(int32_t device, const float *volume, const int32_t resolution[3], float threshold,
+                                    const float mesh_scale[3], float *vertices, int64_t vertex_capacity,
+                                    uint32_t *indices, int64_t index_capacity, int64_t *vertex_count,
+                                    int64_t *index_count, void *stream);
+
 /* The case table in use: triangle_count[256] and edges[256 * 15] (edge ids of
  * each triangle's corners, -1 padded).  Corner k of a cube sits at
  * (k & 1, k >> 1 & 1, k >> 2 & 1); edge 4a + j runs along axis a from the j-th
