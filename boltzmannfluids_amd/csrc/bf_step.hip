@@ -205,11 +205,24 @@ __device__ inline int wrap_index(int v, int n) {
     return (unsigned)v < (unsigned)n ? v : ((v % n) + n) % n;
 }
 
-// One tick of the cells of plane z covered by this block (the body of k_step
-// and of both halves of k_pair).  NTL / NTS: streaming (nontemporal)
+// Where a block works: its first cell is (xb, yb, z) (cells xb + k blockDim.x
+// + threadIdx.x along x, rows yb + threadIdx.y), and its FP64 partial sums go
+// to entry pb.
+struct BlockAt {
+    int xb, yb, z;
+    size_t pb;
+};
+__device__ inline BlockAt grid_block(int cpt, int z) {
+    return BlockAt{(int)(blockIdx.x * cpt * blockDim.x), (int)(blockIdx.y * blockDim.y), z,
+                   blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * (size_t)z)};
+}
+
+// One tick of the cells of the block at `blk` (the body of k_step, of both
+// halves of k_pair and of k_list).  NTL / NTS: streaming (nontemporal)
 // population loads / stores.
 template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, bool NTL, bool NTS>
-__device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict__ partial, const int z) {
+__device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict__ partial, const BlockAt blk) {
+    const int z = blk.z;
     // PER: every axis periodic (and no walls): every neighbour exists, so the
     // open-boundary branches vanish at compile time.
     // CPT cells per thread along x, blockDim.x apart: all their loads are issued
@@ -244,7 +257,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     const int tx = threadIdx.x, ty = threadIdx.y;
     // PER: lanes past the grid address the grid's last row / column (they
     // load in range and store nothing), so no access needs a per-lane guard
-    const int y_in = blockIdx.y * blockDim.y + ty;
+    const int y_in = blk.yb + ty;
     const int y = PER ? min(y_in, Y - 1) : y_in;
 
     // Neighbour coordinates: index 0 = -1, 1 = 0, 2 = +1 along each axis.
@@ -270,7 +283,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     bool in_bounds[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int x_in = (blockIdx.x * CPT + k) * blockDim.x + tx;
+        const int x_in = blk.xb + k * (int)blockDim.x + tx;
         const int x = PER ? min(x_in, X - 1) : x_in;
         const bool w0 = x - 1 < 0, w2 = x + 1 >= X;
         xs[k][0] = w0 ? x - 1 + X : x - 1; xs[k][1] = x; xs[k][2] = w2 ? x + 1 - X : x + 1;
@@ -315,7 +328,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     if constexpr (MP) {
         const int tid = ty * kMpBX + tx;
         const int nplanes = D3 ? 3 : 1;
-        const int x0 = blockIdx.x * kMpBX - 1, y0 = blockIdx.y * kMpBY - 1;
+        const int x0 = blk.xb - 1, y0 = blk.yb - 1;
         for (int k = tid; k < kTileX * kTileY * nplanes; k += kMpBX * kMpBY) {
             const int lx = k % kTileX, ly = (k / kTileX) % kTileY, lz = k / (kTileX * kTileY);
             const int gx = wrap_index(x0 + lx, X);
@@ -785,16 +798,14 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
             const int nw = (blockDim.x * blockDim.y + 63) >> 6;
             double acc = 0.0;
             for (int wv = 0; wv < nw; ++wv) acc += red[wv][tid];
-            // block index within the whole grid (a launch may cover some planes only)
-            const size_t b = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * (size_t)z);
-            partial[b * 5 + tid] = acc;
+            partial[blk.pb * 5 + tid] = acc;
         }
     }
 }
 
 template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER>
 __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
-    step_body<VS, ST, ODD, WALLS, MP, TH, CPT, AL, PER, true, true>(a, partial, a.z0 + (int)blockIdx.z);
+    step_body<VS, ST, ODD, WALLS, MP, TH, CPT, AL, PER, true, true>(a, partial, grid_block(CPT, a.z0 + (int)blockIdx.z));
 }
 
 // A launch of a chunked tick pair: the blocks of the first `nze` planes run
@@ -826,10 +837,10 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs a
                                                                 double *__restrict__ partial) {
     if ((int)blockIdx.z < nze)
         step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
-            ae, nullptr, ae.z0 + (int)blockIdx.z);
+            ae, nullptr, grid_block(CPT, ae.z0 + (int)blockIdx.z));
     else
         step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
-            ao, partial, ao.z0 + (int)blockIdx.z - nze);
+            ao, partial, grid_block(CPT, ao.z0 + (int)blockIdx.z - nze));
 }
 
 // Equilibrium initialisation (set_equilibrium_populations.comp:198-220; the

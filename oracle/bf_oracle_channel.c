@@ -382,6 +382,55 @@ OCH_API double och_gradient(och *s, long T, double *dgamma, double *du_out) {
     return J;
 }
 
+/* The GPU's checkpointed schedule: the forward keeps the state every K ticks;
+ * the adjoint recomputes each segment from its checkpoint.  half_states = 1
+ * keeps the recomputed states j >= 1 of a segment as IEEE binary16 (round to
+ * nearest even of the shifted value, through FP32), and the recompute
+ * continues from the rounded state, as the device does with 16-bit segment
+ * storage; half_states = 0 is exact and equals och_gradient. */
+float orc_half_rne(float v);
+
+OCH_API double och_gradient_checkpointed(och *s, long T, long K, int half_states, double *dgamma, double *du_out) {
+    const long qN = (long)s->q * s->N;
+    const long nseg = (T + K - 1) / K;
+    REAL *ck = (REAL *)malloc(sizeof(REAL) * qN * nseg);
+    REAL *a = (REAL *)malloc(sizeof(REAL) * qN), *b = (REAL *)malloc(sizeof(REAL) * qN);
+    REAL *seg = (REAL *)malloc(sizeof(REAL) * qN * K);
+    rest_state(s, a);
+    memcpy(ck, a, sizeof(REAL) * qN);
+    for (long t = 0; t < T; ++t) {
+        tick(s, a, b);
+        REAL *tmp = a; a = b; b = tmp;
+        if ((t + 1) % K == 0 && t + 1 < T) memcpy(ck + ((t + 1) / K) * qN, a, sizeof(REAL) * qN);
+    }
+    double J = (double)objective(s, a);
+    memcpy(s->state[0], a, sizeof(REAL) * qN);
+    s->cur = 0;
+    REAL *nu = (REAL *)calloc(qN, sizeof(REAL));
+    REAL *nu_prev = (REAL *)calloc(qN, sizeof(REAL));
+    REAL *dg = (REAL *)calloc(s->N, sizeof(REAL));
+    REAL du = 0;
+    objective_adjoint(s, nu);
+    for (long k = nseg - 1; k >= 0; --k) {
+        const long len = (T - k * K) < K ? (T - k * K) : K;
+        memcpy(seg, ck + k * qN, sizeof(REAL) * qN);
+        for (long j = 1; j < len; ++j) {
+            tick(s, seg + (j - 1) * qN, seg + j * qN);
+            if (half_states)
+                for (long e = 0; e < qN; ++e) seg[j * qN + e] = (REAL)orc_half_rne((float)seg[j * qN + e]);
+        }
+        for (long j = len - 1; j >= 0; --j) {
+            memset(nu_prev, 0, sizeof(REAL) * qN);
+            adjoint_tick(s, seg + j * qN, nu, nu_prev, dg, &du);
+            REAL *tmp = nu; nu = nu_prev; nu_prev = tmp;
+        }
+    }
+    for (long c = 0; c < s->N; ++c) dgamma[c] = (double)dg[c];
+    *du_out = (double)du;
+    free(ck); free(a); free(b); free(seg); free(nu); free(nu_prev); free(dg);
+    return J;
+}
+
 /* One tick and its adjoint on arbitrary inputs (shifted values), for the
  * dot-product test: out = tick(src); nu_prev = (d tick / d src)^T nu. */
 OCH_API void och_tick_pair(och *s, const double *src, const double *nu, double *out, double *nu_prev) {

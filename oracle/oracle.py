@@ -72,6 +72,7 @@ def load(precision="f64"):
         "och_precollision": (None, [vp, dp, i32, i32, i32, dp]),
         "och_gradient": (C.c_double, [vp, i64, dp, dp]),
         "och_tick_pair": (None, [vp, dp, dp, dp, dp]),
+        "och_gradient_checkpointed": (C.c_double, [vp, i64, i64, i32, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -267,6 +268,15 @@ class OracleChannel:
         dg = np.zeros(self.cells)
         du = np.zeros(1)
         J = self._lib.och_gradient(self._h, int(steps), _dp(dg), _dp(du))
+        return float(J), dg, float(du[0])
+
+    def gradient_checkpointed(self, steps, checkpoint_every, half_states=False):
+        """The device's schedule: checkpoints every K ticks, segments recomputed
+        (their states kept as shifted FP16 when half_states)."""
+        dg = np.zeros(self.cells)
+        du = np.zeros(1)
+        J = self._lib.och_gradient_checkpointed(self._h, int(steps), int(checkpoint_every), int(bool(half_states)),
+                                                _dp(dg), _dp(du))
         return float(J), dg, float(du[0])
 
     def tick_pair(self, state, nu):
