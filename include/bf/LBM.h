@@ -7,10 +7,12 @@
 // FloatingPointAccuracy, multiphase), set_boundry_properties(...),
 // iterate_time(target_tps), get_total_ticks_elapsed(), the public knobs
 // (velocity_limit, velocity_limit_extreme, is_collide_esoteric, ...).
-// The GL texture getters and render2d_*/render3d_* calls are replaced by host
-// readbacks (get_velocity_density / get_boundries / get_force_temperature),
-// whose layouts match the reference textures (RGBA32F u.xyz+rho, R8 id,
-// RGBA32F F.xyz+T; collide.comp:784-792).
+// The GL texture getters are replaced by host readbacks (get_velocity_density
+// / get_boundries / get_force_temperature), whose layouts match the reference
+// textures (RGBA32F u.xyz+rho, R8 id, RGBA32F F.xyz+T; collide.comp:784-792).
+// The render2d_* / render3d_* calls keep their signatures and hand the field
+// to a renderer that the application registers once (set_field_renderer), so
+// the call sites of the UI viewport and the standalone loop compile unchanged.
 //
 // Link with libbfamd.so.  Uses GLM when it is on the include path, otherwise a
 // minimal stand-in with the few vector types the interface needs.
@@ -19,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <filesystem>
 #include <functional>
 #include <iostream>
 #include <stdexcept>
@@ -239,6 +242,43 @@ public:
         return out;
     }
 
+    // --- rendering (LBM.h:85-95) ------------------------------------------
+    // The reference renders its GL textures with GraphicsCortex programs.  Here
+    // each render call passes the field to the registered renderer, which reads
+    // it with get_velocity_density() / get_boundries() / get_force_temperature()
+    // (e.g. into a glTexImage3D upload) and draws it; `camera` is the caller's
+    // camera object, passed through untouched (null for 2-D calls).
+    enum class RenderedField { density, velocity, boundries, forces, temperature };
+    using FieldRenderer =
+        std::function<void(LBM &solver, RenderedField field, bool three_d, void *camera, int32_t sample_count)>;
+    void set_field_renderer(FieldRenderer renderer) { renderer_ = std::move(renderer); }
+
+    void render2d_density() { render(RenderedField::density, false, nullptr, 0, "render2d_density"); }
+    void render2d_velocity() { render(RenderedField::velocity, false, nullptr, 0, "render2d_velocity"); }
+    void render2d_boundries() { render(RenderedField::boundries, false, nullptr, 0, "render2d_boundries"); }
+    void render2d_forces() { render(RenderedField::forces, false, nullptr, 0, "render2d_forces"); }
+    void render2d_temperature() { render(RenderedField::temperature, false, nullptr, 0, "render2d_temperature"); }
+    template <class Camera>
+    void render3d_density(Camera &camera, int32_t sample_count = 128) {
+        render(RenderedField::density, true, &camera, sample_count, "render3d_density");
+    }
+    template <class Camera>
+    void render3d_velocity(Camera &camera, int32_t sample_count = 128) {
+        render(RenderedField::velocity, true, &camera, sample_count, "render3d_velocity");
+    }
+    template <class Camera>
+    void render3d_boundries(Camera &camera, int32_t sample_count = 128) {
+        render(RenderedField::boundries, true, &camera, sample_count, "render3d_boundries");
+    }
+    template <class Camera>
+    void render3d_forces(Camera &camera, int32_t sample_count = 128) {
+        render(RenderedField::forces, true, &camera, sample_count, "render3d_forces");
+    }
+    template <class Camera>
+    void render3d_temperature(Camera &camera, int32_t sample_count = 128) {
+        render(RenderedField::temperature, true, &camera, sample_count, "render3d_temperature");
+    }
+
     // --- low level accessors (LBM.h:105-120) -----------------------------
     float get_relaxation_time() { return relaxation_time; }
     VelocitySet get_velocity_set() { return velocity_set; }
@@ -263,30 +303,30 @@ public:
     }
     void set_intermolecular_interaction_strength(float value) { intermolecular_interaction_strength = value; }
 
-    // The C handle, for callers that use the C ABI directly (interop, tests).
     // checkpoint files (LBM.h:123-126 of the reference, which declares them
     // without a body): <save_path>/tick_<n>.bfmacro / .bfmeso
-    void save_current_tick_macroscopic(const std::string &save_path) {
+    void save_current_tick_macroscopic(std::filesystem::path save_path) {
         require_solver("save_current_tick_macroscopic");
-        check(bf_save_macroscopic(solver_, save_path.c_str()), "save_current_tick_macroscopic");
+        check(bf_save_macroscopic(solver_, save_path.string().c_str()), "save_current_tick_macroscopic");
     }
-    void save_current_tick_mesoscropic(const std::string &save_path) {
+    void save_current_tick_mesoscropic(std::filesystem::path save_path) {
         require_solver("save_current_tick_mesoscropic");
-        check(bf_save_mesoscopic(solver_, save_path.c_str()), "save_current_tick_mesoscropic");
+        check(bf_save_mesoscopic(solver_, save_path.string().c_str()), "save_current_tick_mesoscropic");
     }
-    void load_tick_macroscopic(const std::string &save_path, int32_t target_tick) {
+    void load_tick_macroscopic(std::filesystem::path save_path, int32_t target_tick) {
         require_solver("load_tick_macroscopic");
-        check(bf_load_macroscopic(solver_, save_path.c_str(), target_tick), "load_tick_macroscopic");
+        check(bf_load_macroscopic(solver_, save_path.string().c_str(), target_tick), "load_tick_macroscopic");
         total_ticks_elapsed = (size_t)target_tick;
         pull_params();
     }
-    void load_tick_mesoscropic(const std::string &save_path, int32_t target_tick) {
+    void load_tick_mesoscropic(std::filesystem::path save_path, int32_t target_tick) {
         require_solver("load_tick_mesoscropic");
-        check(bf_load_mesoscopic(solver_, save_path.c_str(), target_tick), "load_tick_mesoscropic");
+        check(bf_load_mesoscopic(solver_, save_path.string().c_str(), target_tick), "load_tick_mesoscropic");
         total_ticks_elapsed = (size_t)target_tick;
         pull_params();
     }
 
+    // The C handle, for callers that use the C ABI directly (interop, tests).
     bf_solver *native_handle() { return solver_; }
 
     // --- public knobs of the reference (LBM.h:139-145, 242-257) ----------
@@ -344,6 +384,11 @@ private:
                             BF_PARAM_SHAN_CHEN_SYNCHRONOUS};
         for (size_t k = 0; k < sizeof(keys) / sizeof(keys[0]); ++k) check(bf_set_param(solver_, keys[k], values[k], 1), "set_param");
     }
+    void render(RenderedField field, bool three_d, void *camera, int32_t sample_count, const char *what) {
+        require_solver(what);
+        if (!renderer_) fatal(std::string("[LBM Error] ") + what + " is called but no renderer is registered (set_field_renderer)");
+        renderer_(*this, field, three_d, camera, sample_count);
+    }
     std::vector<float> read_floats(int field) {
         require_solver("readback");
         std::vector<float> out(bf_field_bytes(solver_, field) / sizeof(float));
@@ -352,6 +397,7 @@ private:
     }
 
     bf_solver *solver_ = nullptr;
+    FieldRenderer renderer_;
     std::vector<Object> objects_;
     bool first_iteration = true;
     size_t total_ticks_elapsed = 0;

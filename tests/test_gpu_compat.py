@@ -19,8 +19,12 @@ def test_lid_cavity_through_lbm_shim(orc, tmp_path):
     out = tmp_path / "cavity.bin"
     ticks = 400
     print(f"case: LBM.h lid cavity 64x64, {ticks} ticks", flush=True)
-    r = subprocess.run([DEMO, str(out), str(ticks)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DEMO, str(out), str(ticks), str(tmp_path / "ckpt")], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    # the unchanged render calls reached the registered renderer
+    assert "renders 6 bad 0" in r.stdout
+    assert (tmp_path / "ckpt" / f"tick_{ticks}.bfmacro").exists()
     raw = out.read_bytes()
     t = np.frombuffer(raw[:4], np.int32)[0]
     n = 64 * 64
