@@ -338,6 +338,37 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
         }
         __syncthreads();
     }
+    // D3Q27: the weights factor as w1(c_x) w1(c_y) w1(c_z), w1(0) = 2/3,
+    // w1(+-1) = 1/6, so sum_i w_i psi(x + c_i) c_i separates into smoothings S
+    // and differences D along the axes: x (1/6) D_x S_y S_z psi, y (1/6) S_x
+    // D_y S_z psi, z (1/6) S_x S_y D_z psi.  S_y S_z psi, D_y S_z psi and S_y
+    // D_z psi are formed once per block at the 66 x 4 tile columns of its rows;
+    // a cell then reads 8 of them instead of 26 tile entries.
+#ifndef BF_SC_SEPARABLE
+#define BF_SC_SEPARABLE 1  // build-time switch of the separable D3Q27 pseudopotential gradient
+#endif
+    constexpr bool kSep = BF_SC_SEPARABLE && MP && VS == D3Q27;
+    constexpr int kSepN = kTileX * kMpBY;
+    constexpr float k23 = float(2.0 / 3.0), k16 = float(1.0 / 6.0);
+    __shared__ float sep[kSep ? 3 * kSepN : 1];
+    if constexpr (kSep) {
+        constexpr int plane = kTileX * kTileY;
+        for (int k = ty * kMpBX + tx; k < kSepN; k += kMpBX * kMpBY) {
+            const int lx = k % kTileX, ly = k / kTileX + 1;
+            float Pz[3], Dz[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int t = lx + (ly + j - 1) * kTileX;
+                const float pm = tile[t], p0 = tile[t + plane], pp = tile[t + 2 * plane];
+                Pz[j] = __builtin_fmaf(p0, k23, (pm + pp) * k16);
+                Dz[j] = pp - pm;
+            }
+            sep[k] = __builtin_fmaf(Pz[1], k23, (Pz[0] + Pz[2]) * k16);
+            sep[kSepN + k] = Pz[2] - Pz[0];
+            sep[2 * kSepN + k] = __builtin_fmaf(Dz[1], k23, (Dz[0] + Dz[2]) * k16);
+        }
+        __syncthreads();
+    }
 
     // Solid cells skip (collide.comp:696-700).  Even ticks take the solid-neighbour
     // mask (bit 0 = self solid, bit j = neighbour at x + c_j solid), odd ticks
@@ -606,16 +637,27 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
         if constexpr (MP) {
             psi_self = 1.0f - expf(-rho);
             float ax = 0.0f, ay = 0.0f, az = 0.0f;
-            bool ax0 = true, ay0 = true, az0 = true;
+            // a neighbour outside the domain on an open axis takes psi_self
+            // (collide.comp:552-556): such cells keep the direct sum
+            const bool separable = kSep && vx[k][0] && vx[k][2] && vy[0] && vy[2] && vz[0] && vz[2];
+            if (separable) {
+                const int b = ty * kTileX + tx;  // tile column x - 1 of this cell's row
+                ax = (sep[b + 2] - sep[b]) * k16;
+                ay = __builtin_fmaf(sep[kSepN + b + 1], k23, (sep[kSepN + b] + sep[kSepN + b + 2]) * k16) * k16;
+                az = __builtin_fmaf(sep[2 * kSepN + b + 1], k23,
+                                    (sep[2 * kSepN + b] + sep[2 * kSepN + b + 2]) * k16) * k16;
+            } else {
+                bool ax0 = true, ay0 = true, az0 = true;
 #pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
-                const int li = (tx + dx) + (ty + dy) * kTileX + (D3 ? dz : 0) * kTileX * kTileY;
-                const float pn = OK(k, dx, dy, dz) ? tile[li] : psi_self;
-                const float wp = L::w[i] * pn;
-                if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? wp : -wp; ax = ax0 ? t : ax + t; ax0 = false; }
-                if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? wp : -wp; ay = ay0 ? t : ay + t; ay0 = false; }
-                if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? wp : -wp; az = az0 ? t : az + t; az0 = false; }
+                for (int i = 1; i < Q; ++i) {
+                    const int dx = 1 + L::c[i][0], dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
+                    const int li = (tx + dx) + (ty + dy) * kTileX + (D3 ? dz : 0) * kTileX * kTileY;
+                    const float pn = OK(k, dx, dy, dz) ? tile[li] : psi_self;
+                    const float wp = L::w[i] * pn;
+                    if (L::c[i][0] != 0) { float t = L::c[i][0] > 0 ? wp : -wp; ax = ax0 ? t : ax + t; ax0 = false; }
+                    if (L::c[i][1] != 0) { float t = L::c[i][1] > 0 ? wp : -wp; ay = ay0 ? t : ay + t; ay0 = false; }
+                    if (L::c[i][2] != 0) { float t = L::c[i][2] > 0 ? wp : -wp; az = az0 ? t : az + t; az0 = false; }
+                }
             }
             const float kk = -psi_self * a.G;
             Fx += ax * kk; Fy += ay * kk; Fz += az * kk;

@@ -352,6 +352,51 @@ static REAL psi_of(REAL rho) { return (REAL)1 - (REAL)exp(-(double)rho); }
 static REAL psqrt(REAL v) { return (REAL)sqrt((double)v); }
 #endif
 
+#ifdef ORACLE_F32
+/* psi of the previous tick at (x, y, z), periodic axes wrapped. */
+static REAL psi_at(const orc_solver *s, const REAL *p, int x, int y, int z) {
+    int q[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a) wrap(s, a, &q[a]);
+    return p[cid(s, q[0], q[1], q[2])];
+}
+
+/* Every neighbour x + c (|c_a| <= 1) lies inside the domain or wraps on a periodic axis. */
+static int stencil_inside(const orc_solver *s, int x, int y, int z) {
+    const int p[3] = {x, y, z}, n[3] = {s->X, s->Y, s->Z};
+    for (int a = 0; a < 3; ++a)
+        if (!s->per[a] && (p[a] - 1 < 0 || p[a] + 1 >= n[a])) return 0;
+    return 1;
+}
+
+/* sum_i w_i psi(x + c_i) c_i of D3Q27, whose weights factor as w1(c_x) w1(c_y)
+ * w1(c_z) with w1(0) = 2/3 and w1(+-1) = 1/6: with the smoothing S and the
+ * difference D along an axis,
+ *   x: (1/6) D_x S_y S_z psi,   y: (1/6) S_x D_y S_z psi,   z: (1/6) S_x S_y D_z psi.
+ * FP32 build only, in the HIP kernel's operation order (the kernel forms
+ * S_y S_z psi, D_y S_z psi and S_y D_z psi over its LDS tile, then the x
+ * stencils per cell); the FP64 build keeps the direct sum. */
+static void sc_gradient_separable(const orc_solver *s, const REAL *p, int x, int y, int z, REAL *acc) {
+    const REAL k23 = (REAL)(2.0 / 3.0), k16 = (REAL)(1.0 / 6.0);
+    REAL Pzy[3], Dyz[3], Dzy[3];
+    for (int i = 0; i < 3; ++i) {
+        const int xx = x + i - 1;
+        REAL Pz[3], Dz[3];
+        for (int j = 0; j < 3; ++j) {
+            const int yy = y + j - 1;
+            const REAL pm = psi_at(s, p, xx, yy, z - 1), p0 = psi_at(s, p, xx, yy, z), pp = psi_at(s, p, xx, yy, z + 1);
+            Pz[j] = FMA(p0, k23, (pm + pp) * k16);
+            Dz[j] = pp - pm;
+        }
+        Pzy[i] = FMA(Pz[1], k23, (Pz[0] + Pz[2]) * k16);
+        Dyz[i] = Pz[2] - Pz[0];
+        Dzy[i] = FMA(Dz[1], k23, (Dz[0] + Dz[2]) * k16);
+    }
+    acc[0] = (Pzy[2] - Pzy[0]) * k16;
+    acc[1] = FMA(Dyz[1], k23, (Dyz[0] + Dyz[2]) * k16) * k16;
+    acc[2] = FMA(Dzy[1], k23, (Dzy[0] + Dzy[2]) * k16) * k16;
+}
+#endif
+
 /* One tick (collide.comp:676-793). */
 static void tick(orc_solver *s, int update, int save, int want_sums) {
     int odd = (int)(s->t & 1);
@@ -411,6 +456,10 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
         if (s->multiphase) {                             /* collide.comp:534-584 */
             REAL ps = psi_of(rho);
             REAL acc[3] = {0, 0, 0};
+#ifdef ORACLE_F32
+            if (s->vset == V_D3Q27 && stencil_inside(s, x, y, z)) sc_gradient_separable(s, psi_prev, x, y, z, acc);
+            else
+#endif
             for (int i = 1; i < s->q; ++i) {
                 long n;
                 REAL pn = nbr(s, x, y, z, s->T[i].c, +1, &n) ? psi_prev[n] : ps;
