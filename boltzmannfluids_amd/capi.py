@@ -74,7 +74,9 @@ def _share_hip_runtime_with_torch():
 
 
 def load_library(path=None):
-    """Load libbfamd.so and declare every entry point's signature."""
+    """Load libbfamd.so (or the library at `path`) and declare every entry
+    point's signature.  The library loaded last is the one every Solver and
+    module function of this process calls."""
     global _lib
     if _lib is not None and path is None:
         return _lib
@@ -140,8 +142,7 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if path is None:
-        _lib = lib
+    _lib = lib
     return lib
 
 

@@ -811,7 +811,11 @@ bf_status bf_step_timed(bf_solver *s, int64_t n, uint32_t flags, double *elapsed
     bf_status st = run_ticks(s, n, flags);
     if (st != BF_OK) return st;
     BF_HIP(s, hipEventRecord(s->ev1, s->stream));
-    BF_HIP(s, hipEventSynchronize(s->ev1));
+    // poll instead of a blocking wait: the call returns within microseconds of
+    // the last tick rather than after the interrupt-driven wake-up
+    hipError_t hipEventQuery_ev1;
+    while ((hipEventQuery_ev1 = hipEventQuery(s->ev1)) == hipErrorNotReady) {}
+    BF_HIP(s, hipEventQuery_ev1);
     float ms = 0.0f;
     BF_HIP(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
     *elapsed_ms = ms;

@@ -952,6 +952,10 @@ __global__ void __launch_bounds__(256) k_mask(const uint8_t *__restrict__ ids, u
     nmask[c] = m;
 }
 
+#ifndef BF_ROWBUF_16BIT_WALLS
+#define BF_ROWBUF_16BIT_WALLS 0  // build-time switch: row buffer also for 16-bit storage with walls
+#endif
+
 template <int ST, bool ODD>
 hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const LaunchShape &sh,
                           double *partial, hipStream_t stream) {
@@ -960,7 +964,7 @@ hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const
     // The row buffer pays off for the plain kernels and for FP32 walls; with
     // walls and 16-bit storage the direct accesses are faster (the barriers
     // stall blocks whose loads wait on the solid-neighbour mask).
-    const bool al = !ODD && sh.lds_bytes > 0 && !(walls && ST != ST_FP32);
+    const bool al = !ODD && sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
     if (!walls && !mp && !th && a.periodic_x && a.periodic_y && a.periodic_z) {
         if constexpr (!ODD) {
             if (al) {
@@ -1002,7 +1006,7 @@ hipError_t launch_pair_st(const StepArgs &ae, const StepArgs &ao, int nze, int n
     constexpr int VS = BF_VS;
     constexpr int CPT = cells_per_thread(ST, false, false);
     // as for single ticks: the row buffer for the plain kernels and FP32 walls
-    const bool al = sh.lds_bytes > 0 && !(walls && ST != ST_FP32);
+    const bool al = sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
     const dim3 grid(sh.grid.x, sh.grid.y, (unsigned)(nze + nzo));
     const size_t lds = al ? sh.lds_bytes : 0;
 #define BF_P(W, T, A, P)                                                                                   \

@@ -30,6 +30,21 @@ def test_library_exports_every_declared_symbol(bfa):
     assert not missing, missing
 
 
+def test_library_loaded_from_a_path_serves_the_process(bfa):
+    """A library loaded from an explicit path (a variant build) is the one the
+    Solver class and the module functions call afterwards."""
+    from boltzmannfluids_amd import capi
+    default = bfa.load_library()
+    try:
+        other = bfa.load_library(capi.LIB_PATH)
+        assert other is not default
+        assert bfa.load_library() is other
+        assert bfa.velocity_table("D3Q19").shape == (19, 4)
+    finally:
+        capi._lib = default
+    assert bfa.load_library() is default
+
+
 def test_abi_version_and_counts(bfa):
     lib = bfa.load_library()
     assert lib.bf_abi_version() == 1
