@@ -677,13 +677,14 @@ namespace {
 // to its own), so chunk j-2's odd tick can share a launch with chunk j's even
 // tick (run_pair).  The lines the even tick wrote are then still in the
 // 256 MB last-level cache when the odd tick reads them.  Measured on the
-// MI355X (D3Q19, device time per tick): the 384^3 porous duct 1.76 -> 1.59 ms
-// (FP32, chunks of 7 planes) and 1.08 -> 1.05 ms (FP16, 14 planes); the
-// fully periodic 256^3 box gains nothing (FP32 411 -> 419-434 us, FP16 202 ->
-// 224-273 us, chunks of 8-32 planes), although the same schedule of its
-// access pattern alone moves 7.47 instead of 6.35 TB/s.  So the automatic
-// mode pairs the ticks of grids with walls whose lattice exceeds the cache
-// (kPairMinBytes), in chunks of about kPairChunkBytes.
+// MI355X (D3Q19, device time per tick, interleaved on one solver): the 384^3
+// porous duct 1.75 -> 1.60 ms (FP32, chunks of 7 planes; 5 planes 1.66, 10
+// planes 1.70) and 1.09 -> 1.05 ms (FP16, 14 planes); the fully periodic
+// 256^3 box 408.5 -> 399.1 us in FP32 (16 or 20 planes; 12 planes 413, 24
+// planes 418), but 208.7 -> 218-234 us in FP16 (16-48 planes), whose even
+// tick carries the most instructions per byte.  So the automatic mode pairs
+// the ticks of FP32 lattices, and of any lattice with walls, above
+// kPairMinBytes, in chunks of about kPairChunkBytes.
 constexpr double kPairChunkBytes = 80e6;
 constexpr double kPairMinBytes = 384e6;
 
@@ -692,7 +693,7 @@ int pair_chunk(const bf_solver *s) {
     if (s->cfg.multiphase || s->pair_planes < 0) return 0;
     int zc = s->pair_planes;
     if (zc == 0) {
-        if (!s->has_ids) return 0;
+        if (!s->has_ids && s->elem != 4) return 0;
         const double plane = (double)s->X * s->Y * ((double)s->q * s->elem + (s->thermal ? 4.0 * s->qt : 0.0));
         if (plane * s->Z < kPairMinBytes) return 0;
         zc = std::max(1, (int)std::lround(kPairChunkBytes / plane));

@@ -113,8 +113,8 @@ typedef enum bf_param {
      * by chunk of k z-planes, chunk j's even tick in one launch with chunk
      * j-2's odd tick, so that the odd tick reads what the even tick wrote
      * from the last-level cache; 0 = chunks of about 80 MB of lattice for
-     * grids with walls whose lattice exceeds 384 MB (where it was measured
-     * faster), else whole-grid ticks.  Results are
+     * FP32 lattices and for grids with walls whose lattice exceeds 384 MB
+     * (where it was measured faster), else whole-grid ticks.  Results are
      * identical in every mode (each cell sees the same operations); the
      * Shan-Chen solver always runs whole-grid ticks, and so does a grid of
      * fewer than four chunks.  No counterpart in the reference. */

@@ -143,6 +143,26 @@ def test_pairs_equal_whole_grid_ticks_porous_96(bfa, storage):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_automatic_mode_on_the_periodic_tgv_256(bfa, storage):
+    """The automatic mode pairs the ticks of the 256^3 FP32 box (1.3 GB, no
+    walls) and not of the FP16 one; either way the result is bit-identical to
+    whole-grid ticks (6 ticks, saved fields on the last)."""
+    from boltzmannfluids_amd.scenarios import taylor_green
+    n = 256
+    say(f"case: automatic tick pairs TGV {n}^3 {storage}")
+    rho, u = taylor_green(n)
+    out = []
+    for planes in (0, -1):
+        s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19", storage=storage)
+        s.set_param("tick_pair_planes", planes)
+        s.init_fields(rho, u)
+        s.step(6, save=True)
+        out.append((s.read("populations_raw"), s.read("velocity_density")))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_pairs_parameter(bfa):
     s = bfa.Solver((8, 8, 8), 0.6)
     assert s.get_param("tick_pair_planes") == 0
