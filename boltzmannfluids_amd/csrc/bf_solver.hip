@@ -678,13 +678,13 @@ namespace {
 // tick (run_pair).  The lines the even tick wrote are then still in the
 // 256 MB last-level cache when the odd tick reads them.  Measured on the
 // MI355X (D3Q19, device time per tick, interleaved on one solver): the 384^3
-// porous duct 1.75 -> 1.60 ms (FP32, chunks of 7 planes; 5 planes 1.66, 10
-// planes 1.70) and 1.09 -> 1.05 ms (FP16, 14 planes); the fully periodic
-// 256^3 box 408.5 -> 399.1 us in FP32 (16 or 20 planes; 12 planes 413, 24
-// planes 418), but 208.7 -> 218-234 us in FP16 (16-48 planes), whose even
-// tick carries the most instructions per byte.  So the automatic mode pairs
-// the ticks of FP32 lattices, and of any lattice with walls, above
-// kPairMinBytes, in chunks of about kPairChunkBytes.
+// porous duct 1.75 -> 1.58 ms (FP32, chunks of 7 planes; 5 planes 1.66, 10
+// planes 1.70) and 1.08 -> 1.03 ms (FP16, 14 planes); the fully periodic
+// 256^3 box 406 -> 387 us in FP32 (16 planes; 12 planes 413, 24 planes 418),
+// but 208.7 -> 218-234 us in FP16 (16-48 planes), whose even tick carries
+// the most instructions per byte.  So the automatic mode pairs the ticks of
+// FP32 lattices, and of any lattice with walls, above kPairMinBytes, in
+// chunks of about kPairChunkBytes.
 constexpr double kPairChunkBytes = 80e6;
 constexpr double kPairMinBytes = 384e6;
 
@@ -718,12 +718,14 @@ hipError_t dispatch_pair(const bf_solver *s, const StepArgs &ae, const StepArgs 
 
 // An even tick and the odd tick after it, in chunks of zc planes (m >= 4
 // chunks), on the handle's stream:
-//   E0 | E1 | E2 | E3 + O1 | ... | E(m-1) + O(m-3) | O(m-2) + O(m-1) | O0
+//   E0 E1 E2 | E3 + O1 | ... | E(m-1) + O(m-3) | O(m-2) O(m-1) O0
 // The odd tick of chunk l needs the even ticks of chunks l-1, l and l+1 (the
 // even tick writes into the planes beside its own; chunks 0 and m-1 are
 // neighbours on a periodic z axis), all of them in earlier launches; within a
 // launch the even chunk j and the odd chunk j-2 are at least one plane apart.
-// `flags` apply to the odd tick.
+// The first launch holds three even chunks and the last three odd chunks
+// (their planes wrap from Z-1 to 0): even ticks of different cells, or odd
+// ticks, never touch one another's slots.  `flags` apply to the odd tick.
 bf_status run_pair(bf_solver *s, int zc, uint32_t flags) {
     const int Z = s->Z, m = (Z + zc - 1) / zc;
     StepArgs ae = make_args(s, true), ao = make_args(s, true);
@@ -731,19 +733,18 @@ bf_status run_pair(bf_solver *s, int zc, uint32_t flags) {
     double *partial = (flags & BF_STEP_CONSERVATION) ? s->partial : nullptr;
     auto lo = [&](int j) { return j * zc; };
     auto hi = [&](int j) { return std::min(Z, (j + 1) * zc); };
-    // even tick on chunk je (none if je < 0), odd tick on chunks [jo0, jo1)
-    auto launch = [&](int je, int jo0, int jo1) -> bf_status {
-        int nze = 0, nzo = 0;
-        if (je >= 0) { ae.z0 = lo(je); nze = hi(je) - lo(je); }
-        if (jo0 < jo1) { ao.z0 = lo(jo0); nzo = hi(jo1 - 1) - lo(jo0); }
+    // even tick on planes [ze0, ze0 + nze), odd tick on planes zo0, zo0 + 1, ...
+    // (nzo of them, modulo Z)
+    auto launch = [&](int ze0, int nze, int zo0, int nzo) -> bf_status {
+        ae.z0 = ze0;
+        ao.z0 = zo0;
         const hipError_t e = dispatch_pair(s, ae, ao, nze, nzo, partial);
         if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] tick-pair launch failed: %s", hipGetErrorString(e));
         return BF_OK;
     };
-    bf_status st = BF_OK;
-    for (int j = 0; j < m && st == BF_OK; ++j) st = j >= 3 ? launch(j, j - 2, j - 1) : launch(j, 0, 0);
-    if (st == BF_OK) st = launch(-1, m - 2, m);
-    if (st == BF_OK) st = launch(-1, 0, 1);
+    bf_status st = launch(0, hi(2), 0, 0);
+    for (int j = 3; j < m && st == BF_OK; ++j) st = launch(lo(j), hi(j) - lo(j), lo(j - 2), hi(j - 2) - lo(j - 2));
+    if (st == BF_OK) st = launch(0, 0, lo(m - 2), Z - lo(m - 2) + hi(0));
     if (st != BF_OK) return st;
     if (partial) {
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);

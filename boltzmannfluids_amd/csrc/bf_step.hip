@@ -892,12 +892,15 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs a
     // planes of the two chunks measured 2-8 % slower)
     const bool even = (int)blockIdx.z < nze;
     const int zl = even ? (int)blockIdx.z : (int)blockIdx.z - nze;
-    if (even)
+    if (even) {
         step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
             ae, nullptr, grid_block(CPT, ae.z0 + zl));
-    else
+    } else {
+        // the odd planes may wrap from Z - 1 to 0 (the last launch of a pair)
+        const int z = ao.z0 + zl;
         step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
-            ao, partial, grid_block(CPT, ao.z0 + zl));
+            ao, partial, grid_block(CPT, z < ao.Z ? z : z - ao.Z));
+    }
 }
 
 // Equilibrium initialisation (set_equilibrium_populations.comp:198-220; the
