@@ -677,15 +677,16 @@ namespace {
 // to its own), so chunk j-2's odd tick can share a launch with chunk j's even
 // tick (run_pair).  The lines the even tick wrote are then still in the
 // 256 MB last-level cache when the odd tick reads them.  Measured on the
-// MI355X (D3Q19, device time per tick, interleaved on one solver): the 384^3
-// porous duct 1.75 -> 1.58 ms (FP32, chunks of 7 planes; 5 planes 1.66, 10
-// planes 1.70) and 1.08 -> 1.03 ms (FP16, 14 planes); the fully periodic
-// 256^3 box 406 -> 387 us in FP32 (16 planes; 12 planes 413, 24 planes 418),
-// but 208.7 -> 218-234 us in FP16 (16-48 planes), whose even tick carries
-// the most instructions per byte.  So the automatic mode pairs the ticks of
-// FP32 lattices, and of any lattice with walls, above kPairMinBytes, in
-// chunks of about kPairChunkBytes.
-constexpr double kPairChunkBytes = 80e6;
+// MI355X (D3Q19, device time per tick, interleaved on one solver, ranges
+// over boxes): the 384^3 porous duct 1.74-1.75 -> 1.575-1.586 ms (FP32,
+// chunks of 8 planes; 6 planes 1.61-1.62, 10 planes 1.70) and 1.08 ->
+// 1.03-1.04 ms (FP16, 14-16 planes); the fully periodic 256^3 box
+// 406-407 -> 390-393 us in FP32 (18 planes; 16 planes 387-399, 12 planes 413,
+// 24 planes 418), but 208.7 -> 213-234 us in FP16 (16-48 planes), whose even
+// tick carries the most instructions per byte.  So the automatic mode pairs
+// the ticks of FP32 lattices, and of any lattice with walls, above
+// kPairMinBytes, in chunks of about kPairChunkBytes.
+constexpr double kPairChunkBytes = 90e6;
 constexpr double kPairMinBytes = 384e6;
 
 // planes per chunk, or 0 for whole-grid ticks

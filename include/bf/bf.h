@@ -112,7 +112,7 @@ typedef enum bf_param {
      * after it.  -1 = each tick over the whole grid; k > 0 = the pair chunk
      * by chunk of k z-planes, chunk j's even tick in one launch with chunk
      * j-2's odd tick, so that the odd tick reads what the even tick wrote
-     * from the last-level cache; 0 = chunks of about 80 MB of lattice for
+     * from the last-level cache; 0 = chunks of about 90 MB of lattice for
      * FP32 lattices and for grids with walls whose lattice exceeds 384 MB
      * (where it was measured faster), else whole-grid ticks.  Results are
      * identical in every mode (each cell sees the same operations); the
