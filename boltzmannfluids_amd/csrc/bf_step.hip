@@ -265,10 +265,11 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     // loads and cost the scalar-base addressing of every access.
     const bool wy0 = y - 1 < 0, wy2 = y + 1 >= Y;
     const int ys[3] = {wy0 ? y - 1 + Y : y - 1, y, wy2 ? y + 1 - Y : y + 1};
-    const bool vy[3] = {!wy0 || a.periodic_y != 0, true, !wy2 || a.periodic_y != 0};
+    // (the fully periodic kernel runs only on grids periodic on every axis)
+    const bool vy[3] = {PER || !wy0 || a.periodic_y != 0, true, PER || !wy2 || a.periodic_y != 0};
     const bool wz0 = D3 && z - 1 < 0, wz2 = D3 && z + 1 >= Z;
     const int zs[3] = {D3 ? (wz0 ? z - 1 + Z : z - 1) : z, z, D3 ? (wz2 ? z + 1 - Z : z + 1) : z};
-    const bool vz[3] = {!wz0 || a.periodic_z != 0, true, !wz2 || a.periodic_z != 0};
+    const bool vz[3] = {PER || !wz0 || a.periodic_z != 0, true, PER || !wz2 || a.periodic_z != 0};
     const int ry[3] = {ys[0] * X, ys[1] * X, ys[2] * X};
     const int XY = X * Y;
     const int pz[3] = {zs[0] * XY, zs[1] * XY, zs[2] * XY};
@@ -287,7 +288,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
         const int x = PER ? min(x_in, X - 1) : x_in;
         const bool w0 = x - 1 < 0, w2 = x + 1 >= X;
         xs[k][0] = w0 ? x - 1 + X : x - 1; xs[k][1] = x; xs[k][2] = w2 ? x + 1 - X : x + 1;
-        vx[k][0] = !w0 || a.periodic_x != 0; vx[k][1] = true; vx[k][2] = !w2 || a.periodic_x != 0;
+        vx[k][0] = PER || !w0 || a.periodic_x != 0; vx[k][1] = true; vx[k][2] = PER || !w2 || a.periodic_x != 0;
         self[k] = x + ry[1] + pz[1];
         active[k] = x_in < X && y_in < Y;
         in_bounds[k] = active[k];   // in the grid (active drops solid cells below)
