@@ -45,6 +45,8 @@ struct StepArgs {
     float *gslot[kMaxQT];    // base of thermal slot q (g + q * stride)
     int X, Y, Z;
     int z0;                  // first plane of the launch (grid.z planes from z0)
+    int bdx, bdy;            // block shape of the launch (set by the launcher; scalar loads
+                             // instead of the dispatch packet's 16-bit group size)
     int periodic_x, periodic_y, periodic_z;
     const uint8_t *ids;      // boundary id per cell (or null)
     const uint32_t *nmask;   // solid-neighbour mask per cell (see launch_mask), with ids

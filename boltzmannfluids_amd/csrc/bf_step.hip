@@ -205,15 +205,15 @@ __device__ inline int wrap_index(int v, int n) {
     return (unsigned)v < (unsigned)n ? v : ((v % n) + n) % n;
 }
 
-// Where a block works: its first cell is (xb, yb, z) (cells xb + k blockDim.x
+// Where a block works: its first cell is (xb, yb, z) (cells xb + k a.bdx
 // + threadIdx.x along x, rows yb + threadIdx.y), and its FP64 partial sums go
 // to entry pb.
 struct BlockAt {
     int xb, yb, z;
     size_t pb;
 };
-__device__ inline BlockAt grid_block(int cpt, int z) {
-    return BlockAt{(int)(blockIdx.x * cpt * blockDim.x), (int)(blockIdx.y * blockDim.y), z,
+__device__ inline BlockAt grid_block(const StepArgs &a, int cpt, int z) {
+    return BlockAt{(int)blockIdx.x * cpt * a.bdx, (int)blockIdx.y * a.bdy, z,
                    blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * (size_t)z)};
 }
 
@@ -284,7 +284,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     bool in_bounds[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int x_in = blk.xb + k * (int)blockDim.x + tx;
+        const int x_in = blk.xb + k * a.bdx + tx;
         const int x = PER ? min(x_in, X - 1) : x_in;
         const bool w0 = x - 1 < 0, w2 = x + 1 >= X;
         xs[k][0] = w0 ? x - 1 + X : x - 1; xs[k][1] = x; xs[k][2] = w2 ? x + 1 - X : x + 1;
@@ -841,7 +841,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
             s_t += __shfl_xor(s_t, off);
         }
         __shared__ double red[8][5];
-        const int tid = ty * blockDim.x + tx;
+        const int tid = ty * a.bdx + tx;
         const int wave = tid >> 6, lane = tid & 63;
         if (lane == 0) {
             red[wave][0] = s_rho; red[wave][1] = s_mx; red[wave][2] = s_my;
@@ -849,7 +849,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
         }
         __syncthreads();
         if (tid < 5) {
-            const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+            const int nw = (a.bdx * a.bdy + 63) >> 6;
             double acc = 0.0;
             for (int wv = 0; wv < nw; ++wv) acc += red[wv][tid];
             partial[blk.pb * 5 + tid] = acc;
@@ -859,7 +859,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
 
 template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER>
 __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
-    step_body<VS, ST, ODD, WALLS, MP, TH, CPT, AL, PER, true, true>(a, partial, grid_block(CPT, a.z0 + (int)blockIdx.z));
+    step_body<VS, ST, ODD, WALLS, MP, TH, CPT, AL, PER, true, true>(a, partial, grid_block(a, CPT, a.z0 + (int)blockIdx.z));
 }
 
 // A launch of a chunked tick pair: the blocks of the first `nze` planes run
@@ -895,12 +895,12 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs a
     const int zl = even ? (int)blockIdx.z : (int)blockIdx.z - nze;
     if (even) {
         step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
-            ae, nullptr, grid_block(CPT, ae.z0 + zl));
+            ae, nullptr, grid_block(ae, CPT, ae.z0 + zl));
     } else {
         // the odd planes may wrap from Z - 1 to 0 (the last launch of a pair)
         const int z = ao.z0 + zl;
         step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
-            ao, partial, grid_block(CPT, z < ao.Z ? z : z - ao.Z));
+            ao, partial, grid_block(ao, CPT, z < ao.Z ? z : z - ao.Z));
     }
 }
 
@@ -976,9 +976,12 @@ __global__ void __launch_bounds__(256) k_mask(const uint8_t *__restrict__ ids, u
 #endif
 
 template <int ST, bool ODD>
-hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const LaunchShape &sh,
+hipError_t launch_step_st(const StepArgs &a_in, bool walls, bool mp, bool th, const LaunchShape &sh,
                           double *partial, hipStream_t stream) {
     constexpr int VS = BF_VS;
+    StepArgs a = a_in;
+    a.bdx = (int)sh.block.x;
+    a.bdy = (int)sh.block.y;
     constexpr int CPT = cells_per_thread(ST, false, false);
     // The row buffer pays off for the plain kernels and for FP32 walls; with
     // walls and 16-bit storage the direct accesses are faster (the barriers
@@ -1020,9 +1023,12 @@ hipError_t launch_step_st(const StepArgs &a, bool walls, bool mp, bool th, const
 }
 
 template <int ST>
-hipError_t launch_pair_st(const StepArgs &ae, const StepArgs &ao, int nze, int nzo, bool walls, bool th,
+hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, int nze, int nzo, bool walls, bool th,
                           const LaunchShape &sh, double *partial, hipStream_t stream) {
     constexpr int VS = BF_VS;
+    StepArgs ae = ae_in, ao = ao_in;
+    ae.bdx = ao.bdx = (int)sh.block.x;
+    ae.bdy = ao.bdy = (int)sh.block.y;
     constexpr int CPT = cells_per_thread(ST, false, false);
     // as for single ticks: the row buffer for the plain kernels and FP32 walls
     const bool al = sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
