@@ -48,6 +48,11 @@ struct StepArgs {
     int bdx, bdy;            // block shape of the launch (set by the launcher; scalar loads
                              // instead of the dispatch packet's 16-bit group size)
     int periodic_x, periodic_y, periodic_z;
+    // Every neighbour of every cell that updates lies in the grid: the axes
+    // are periodic, or every cell on the two boundary planes of a
+    // non-periodic axis is solid (checked on the host from the ids).  The
+    // launcher then runs the kernels without neighbour checks.
+    int closed;
     const uint8_t *ids;      // boundary id per cell (or null)
     const uint32_t *nmask;   // solid-neighbour mask per cell (see launch_mask), with ids
     const ObjectEntry *objects;  // 256 entries
@@ -57,6 +62,10 @@ struct StepArgs {
     float fx, fy, fz;        // constant force
     const float4 *force;     // per-cell force (xyz) when forcing == 2
     float tau;
+    // 1 / tau and the Guo prefactor 1 - 1 / (2 tau), rounded once on the host
+    // as the kernel would (IEEE division): a division in the kernel is a
+    // dozen vector instructions per wave, although its operand is uniform
+    float inv_tau, guo_pre;
     float ulim, uext;        // velocity clamp / equilibrium reset thresholds
     int source_on;           // Guo source active (forcing || multiphase || thermal)
     // Shan-Chen
@@ -66,6 +75,7 @@ struct StepArgs {
     int write_psi;
     // thermal
     float tau_t, beta, t_ref, cs2_t;
+    float inv_tau_t, k1_t, k2_t;  // 1 / tau_t, 1 / cs2_t, k1_t^2 / 2 (host-rounded, as inv_tau)
     // outputs
     int update;              // write populations
     float4 *vd;              // (u, rho) when non-null

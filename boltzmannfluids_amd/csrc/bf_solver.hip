@@ -65,6 +65,7 @@ struct bf_solver {
     int pair_planes = 0;  // chunked tick pairs: 0 automatic, -1 off, k > 0 chunks of k planes
     float fconst[3] = {0, 0, 0};
     int forcing = 0, thermal = 0, has_ids = 0, initialized = 0;
+    int closed = 0;  // see StepArgs::closed
     int cur = 0;
     int64_t ticks = 0;
     int64_t tick_base = 0;  // tick count at the last equilibrium initialisation: AA parity = ticks - tick_base
@@ -213,6 +214,34 @@ hipError_t dispatch_step(const bf_solver *s, const StepArgs &a, bool odd, double
     return hipErrorInvalidValue;
 }
 
+// StepArgs::closed from the host ids (cell order, or null for no boundaries):
+// every axis periodic, or each non-periodic axis solid on both of its boundary
+// planes, so that no cell that updates has a neighbour outside the grid.  The
+// z axis of a 2-D velocity set has no neighbours and is ignored.
+int closed_domain(const bf_solver *s, const uint8_t *ids) {
+    const int64_t n[3] = {s->X, s->Y, s->Z};
+    const int dims = (s->vs == D2Q9) ? 2 : 3;
+    bool all_periodic = true;
+    for (int d = 0; d < dims; ++d) all_periodic = all_periodic && s->cfg.periodic[d] != 0;
+    if (all_periodic) return 1;
+    if (!ids) return 0;
+    for (int d = 0; d < dims; ++d) {
+        if (s->cfg.periodic[d]) continue;
+        // the two planes coord_d = 0 and coord_d = n[d] - 1
+        const int a1 = (d + 1) % 3, a2 = (d + 2) % 3;
+        for (int64_t side = 0; side < 2; ++side) {
+            const int64_t cd = side ? n[d] - 1 : 0;
+            for (int64_t j = 0; j < n[a2]; ++j)
+                for (int64_t i = 0; i < n[a1]; ++i) {
+                    int64_t co[3];
+                    co[d] = cd; co[a1] = i; co[a2] = j;
+                    if (ids[co[0] + n[0] * (co[1] + n[1] * co[2])] == 0) return 0;
+                }
+        }
+    }
+    return 1;
+}
+
 hipError_t dispatch_mask(const bf_solver *s) {
     const int px = s->cfg.periodic[0] != 0, py = s->cfg.periodic[1] != 0, pz = s->cfg.periodic[2] != 0;
     switch (s->vs) {
@@ -269,6 +298,7 @@ StepArgs make_args(bf_solver *s, bool update) {
     a.periodic_x = s->cfg.periodic[0] != 0;
     a.periodic_y = s->cfg.periodic[1] != 0;
     a.periodic_z = s->cfg.periodic[2] != 0;
+    a.closed = s->closed;
     a.ids = s->ids;
     a.nmask = s->nmask;
     a.objects = s->objects_d;
@@ -281,6 +311,8 @@ StepArgs make_args(bf_solver *s, bool update) {
     a.fx = s->fconst[0]; a.fy = s->fconst[1]; a.fz = s->fconst[2];
     a.force = s->force_d;
     a.tau = s->tau;
+    a.inv_tau = 1.0f / s->tau;
+    a.guo_pre = 1.0f - 1.0f / (2.0f * s->tau);
     a.ulim = s->ulim; a.uext = s->uext;
     a.source_on = (s->forcing != 0 || s->cfg.multiphase || s->thermal) ? 1 : 0;
     a.G = s->G;
@@ -290,6 +322,9 @@ StepArgs make_args(bf_solver *s, bool update) {
     }
     a.write_psi = 1;
     a.tau_t = s->tau_t; a.beta = s->beta; a.t_ref = s->t_ref; a.cs2_t = s->cs2_t;
+    a.inv_tau_t = 1.0f / s->tau_t;
+    a.k1_t = 1.0f / s->cs2_t;
+    a.k2_t = 0.5f * a.k1_t * a.k1_t;
     a.update = update ? 1 : 0;
     return a;
 }
@@ -524,6 +559,7 @@ bf_status bf_init_fields(bf_solver *s, const float *rho, const float *u, const f
             BF_HIP(s, dispatch_mask(s));
         }
     }
+    s->closed = closed_domain(s, s->has_ids ? boundary_id : nullptr);
     // force detection (LBM.cpp:531-541,562)
     dfree(s->force_d); s->force_d = nullptr;
     s->forcing = 0;
@@ -1324,6 +1360,7 @@ bf_status bf_load_mesoscopic(bf_solver *s, const char *directory, int64_t tick) 
     BF_HIP(s, hipMemcpy(s->ft, ft.data(), ft.size(), hipMemcpyHostToDevice));
     apply_params(s, h);
     s->has_ids = h.has_ids;
+    s->closed = closed_domain(s, h.has_ids ? ids.data() : nullptr);
     s->thermal = h.thermal;
     s->forcing = h.forcing;
     s->cur = h.cur;

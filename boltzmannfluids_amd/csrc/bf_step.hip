@@ -223,8 +223,10 @@ __device__ inline BlockAt grid_block(const StepArgs &a, int cpt, int z) {
 template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, bool NTL, bool NTS>
 __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict__ partial, const BlockAt blk) {
     const int z = blk.z;
-    // PER: every axis periodic (and no walls): every neighbour exists, so the
-    // open-boundary branches vanish at compile time.
+    // PER: the grid is closed (every axis periodic, or solid boundary planes
+    // on the others): every neighbour of a cell that updates exists, so the
+    // open-boundary branches vanish at compile time.  Without walls every
+    // cell of the grid updates (kEvery).
     // CPT cells per thread along x, blockDim.x apart: all their loads are issued
     // before any of them is updated.
     // AL (even ticks, blocks spanning whole x-rows): the accesses of directions
@@ -235,7 +237,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     // cache lines (the periodic x wrap stays inside the block).
     static_assert(CPT == 1 || (!MP && !TH), "multi-cell threads only for the plain and wall kernels");
     static_assert(!(AL && (ODD || MP)), "aligned rows only for the even tick of non-multiphase kernels");
-    static_assert(!(PER && (WALLS || MP || TH)), "the periodic specialisation is the plain kernel");
+    constexpr bool kEvery = PER && !WALLS;
     extern __shared__ float lds_rows[];
     using L = Lattice<VS>;
     using S = Store<ST, NTL, NTS>;
@@ -270,9 +272,9 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     const bool wz0 = D3 && z - 1 < 0, wz2 = D3 && z + 1 >= Z;
     const int zs[3] = {D3 ? (wz0 ? z - 1 + Z : z - 1) : z, z, D3 ? (wz2 ? z + 1 - Z : z + 1) : z};
     const bool vz[3] = {PER || !wz0 || a.periodic_z != 0, true, PER || !wz2 || a.periodic_z != 0};
-    const int ry[3] = {ys[0] * X, ys[1] * X, ys[2] * X};
+    int ry[3] = {ys[0] * X, ys[1] * X, ys[2] * X};
     const int XY = X * Y;
-    const int pz[3] = {zs[0] * XY, zs[1] * XY, zs[2] * XY};
+    int pz[3] = {zs[0] * XY, zs[1] * XY, zs[2] * XY};
 
     // neighbour (dx, dy, dz) of cell k inside the domain (indices 0/1/2 = -1/0/+1)
 #define OK(k, dx, dy, dz) (PER || (vx[k][dx] && vy[dy] && vz[dz]))
@@ -426,7 +428,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
                 raw[k][i] = S::raw(SLOT(i), idx);
             }
         }
-        const bool act = PER || active[k];
+        const bool act = kEvery || active[k];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int o = opposite(i);
@@ -602,6 +604,15 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
 #pragma unroll
         for (int k = 0; k < CPT; ++k) asm volatile("" : "+v"(nm[k]));
     }
+    // Closed wall kernels: the neighbour coordinates pass through an empty asm
+    // too, so that the stores compute their neighbours' elements afresh.
+    // Without the neighbour checks the compiler otherwise keeps the load
+    // phase's 18 elements live across the collision and spills to memory.
+    if constexpr (WALLS && PER && !ODD) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) asm volatile("" : "+v"(xs[k][0]), "+v"(xs[k][1]), "+v"(xs[k][2]));
+        asm volatile("" : "+v"(ry[0]), "+v"(ry[1]), "+v"(ry[2]), "+v"(pz[0]), "+v"(pz[1]), "+v"(pz[2]));
+    }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         if (!active[k]) continue;
@@ -692,8 +703,8 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
             const float uu = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
             const float fu = __builtin_fmaf(Fz, uz, __builtin_fmaf(Fy, uy, Fx * ux));
             const float c1 = -1.5f * uu;
-            const float inv_tau = 1.0f / a.tau;
-            const float pre = 1.0f - 1.0f / (2.0f * a.tau);
+            const float inv_tau = a.inv_tau;
+            const float pre = a.guo_pre;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 const float w = L::w[i];
@@ -761,9 +772,9 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
         if constexpr (TH) {
             if (a.update) {
                 const float uu = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
-                const float k1 = 1.0f / a.cs2_t, k2 = 0.5f * k1 * k1;
+                const float k1 = a.k1_t, k2 = a.k2_t;
                 const float t1 = __builtin_fmaf(-0.5f * k1, uu, 1.0f);
-                const float inv_tau_t = 1.0f / a.tau_t;
+                const float inv_tau_t = a.inv_tau_t;
 #pragma unroll
                 for (int q = 0; q < QT; ++q) {
                     const int o = opposite(q);
@@ -810,9 +821,9 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
                 const int sx = 1 - L::c[i][0];             // the writer sits at x - c_x
                 if (!(PER || vx[k][sx])) continue;
                 const float v = LDS(i, x);
-                // (the periodic kernel has no empty entries: every cell of the
-                // grid updates and no write leaves the domain)
-                if (!PER && __float_as_uint(v) == kNoValue) continue;
+                // (the periodic plain kernel has no empty entries: every cell
+                // of the grid updates and no write leaves the domain)
+                if (!kEvery && __float_as_uint(v) == kNoValue) continue;
                 const int dy = 1 + L::c[i][1], dz = 1 + L::c[i][2];
                 S::store(SLOT(opposite(i)), ROW(k, dy, dz), v);
             }
@@ -987,38 +998,36 @@ hipError_t launch_step_st(const StepArgs &a_in, bool walls, bool mp, bool th, co
     // walls and 16-bit storage the direct accesses are faster (the barriers
     // stall blocks whose loads wait on the solid-neighbour mask).
     const bool al = !ODD && sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
-    if (!walls && !mp && !th && a.periodic_x && a.periodic_y && a.periodic_z) {
-        if constexpr (!ODD) {
-            if (al) {
-                hipLaunchKernelGGL((k_step<VS, ST, ODD, false, false, false, CPT, true, true>), sh.grid, sh.block,
-                                   sh.lds_bytes, stream, a, partial);
-                return hipGetLastError();
-            }
-        }
-        hipLaunchKernelGGL((k_step<VS, ST, ODD, false, false, false, CPT, false, true>), sh.grid, sh.block, 0, stream,
-                           a, partial);
-        return hipGetLastError();
-    }
-#define BF_L(W, M, T)                                                                                      \
+    // Closed grids run the kernels without neighbour checks: the plain kernel
+    // on a periodic grid, and the wall kernels (plain walls, thermal walls,
+    // Shan-Chen with thermal walls) where solid planes close the grid.
+#define BF_L(W, M, T, P)                                                                                   \
     do {                                                                                                   \
         if constexpr (!ODD && !M) {                                                                        \
             if (al) {                                                                                      \
-                hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, T ? 1 : CPT, true, false>), sh.grid, sh.block, \
+                hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, true, P>), sh.grid, sh.block, \
                                    sh.lds_bytes, stream, a, partial);                                      \
                 return hipGetLastError();                                                                  \
             }                                                                                              \
         }                                                                                                  \
-        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, false, false>), sh.grid, sh.block, 0, \
+        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, false, P>), sh.grid, sh.block, 0, \
                            stream, a, partial);                                                            \
         return hipGetLastError();                                                                          \
     } while (0)
+    const bool closed = a.closed != 0;
     if (walls) {
-        if (mp) { if (th) BF_L(true, true, true); else BF_L(true, true, false); }
-        else { if (th) BF_L(true, false, true); else BF_L(true, false, false); }
-    } else {
-        if (mp) { if (th) BF_L(false, true, true); else BF_L(false, true, false); }
-        else { if (th) BF_L(false, false, true); else BF_L(false, false, false); }
+        if (mp) {
+            if (th) { if (closed) BF_L(true, true, true, true); BF_L(true, true, true, false); }
+            BF_L(true, true, false, false);
+        }
+        if (th) { if (closed) BF_L(true, false, true, true); BF_L(true, false, true, false); }
+        if (closed) BF_L(true, false, false, true);
+        BF_L(true, false, false, false);
     }
+    if (mp) { if (th) BF_L(false, true, true, false); else BF_L(false, true, false, false); }
+    if (th) BF_L(false, false, true, false);
+    if (closed) BF_L(false, false, false, true);
+    BF_L(false, false, false, false);
 #undef BF_L
 }
 
@@ -1040,16 +1049,19 @@ hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, int nze,
                            partial);                                                                       \
         return hipGetLastError();                                                                          \
     } while (0)
-    if (!walls && !th && ae.periodic_x && ae.periodic_y && ae.periodic_z) {
-        if (al) BF_P(false, false, true, true);
-        BF_P(false, false, false, true);
-    }
+    const bool closed = ae.closed != 0;
     if (walls) {
-        if (th) { if (al) BF_P(true, true, true, false); BF_P(true, true, false, false); }
+        if (th) {
+            if (closed) { if (al) BF_P(true, true, true, true); BF_P(true, true, false, true); }
+            if (al) BF_P(true, true, true, false);
+            BF_P(true, true, false, false);
+        }
+        if (closed) { if (al) BF_P(true, false, true, true); BF_P(true, false, false, true); }
         if (al) BF_P(true, false, true, false);
         BF_P(true, false, false, false);
     }
     if (th) { if (al) BF_P(false, true, true, false); BF_P(false, true, false, false); }
+    if (closed) { if (al) BF_P(false, false, true, true); BF_P(false, false, false, true); }
     if (al) BF_P(false, false, true, false);
     BF_P(false, false, false, false);
 #undef BF_P
