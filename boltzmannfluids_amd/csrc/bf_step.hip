@@ -29,6 +29,8 @@
 namespace bf {
 namespace {
 
+constexpr int kSrcOff = 0, kSrcOn = 1, kSrcRuntime = 2;  // Guo source policy of a kernel (SRC)
+
 constexpr int kMpBX = 64;   // multiphase tile: 64 x 4 cells, one z-plane
 constexpr int kMpBY = 4;
 constexpr int kTileX = kMpBX + 2, kTileY = kMpBY + 2;
@@ -220,7 +222,8 @@ __device__ inline BlockAt grid_block(const StepArgs &a, int cpt, int z) {
 // One tick of the cells of the block at `blk` (the body of k_step, of both
 // halves of k_pair and of k_list).  NTL / NTS: streaming (nontemporal)
 // population loads / stores.
-template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, bool NTL, bool NTS>
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, int SRC, bool NTL,
+          bool NTS>
 __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict__ partial, const BlockAt blk) {
     const int z = blk.z;
     // PER: the grid is closed (every axis periodic, or solid boundary planes
@@ -238,6 +241,12 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     static_assert(CPT == 1 || (!MP && !TH), "multi-cell threads only for the plain and wall kernels");
     static_assert(!(AL && (ODD || MP)), "aligned rows only for the even tick of non-multiphase kernels");
     constexpr bool kEvery = PER && !WALLS;
+    // SRC: the Guo source (a force, Shan-Chen or a thermal lattice) is off
+    // (kSrcOff), on (kSrcOn), or decided per launch (kSrcRuntime).  The
+    // launch's flag tested per direction is compiled as the source of every
+    // direction computed and selected away: 7-8 vector instructions per
+    // direction without a force.
+    static_assert(SRC == kSrcOn || !(MP || TH), "the Shan-Chen and thermal kernels always have the source on");
     extern __shared__ float lds_rows[];
     using L = Lattice<VS>;
     using S = Store<ST, NTL, NTS>;
@@ -720,7 +729,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
                 // h - (h - h_eq)/tau + Guo source, or h_eq past the reset
                 // threshold (a select, not a per-direction branch)
                 v = __builtin_fmaf(e - fi[k][i], inv_tau, fi[k][i]);
-                if (a.source_on) {
+                if (SRC == kSrcOn || (SRC == kSrcRuntime && a.source_on)) {
                     const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
                     v += (pre * w) * (__builtin_fmaf(9.0f * cf, cu, 3.0f * cf) - 3.0f * fu);
                 }
@@ -868,9 +877,10 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     }
 }
 
-template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER>
+template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, int SRC>
 __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, double *__restrict__ partial) {
-    step_body<VS, ST, ODD, WALLS, MP, TH, CPT, AL, PER, true, true>(a, partial, grid_block(a, CPT, a.z0 + (int)blockIdx.z));
+    step_body<VS, ST, ODD, WALLS, MP, TH, CPT, AL, PER, SRC, true, true>(a, partial,
+                                                                      grid_block(a, CPT, a.z0 + (int)blockIdx.z));
 }
 
 // A launch of a chunked tick pair: the blocks of the first `nze` planes run
@@ -897,7 +907,7 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
 #define BF_PAIR_ODD_NT_STORE 1
 #endif
 #define BF_OCCUPANCY_PAIR __attribute__((amdgpu_waves_per_eu(TH ? 1 : BF_WAVES_PER_EU)))
-template <int VS, int ST, bool WALLS, bool TH, int CPT, bool AL, bool PER>
+template <int VS, int ST, bool WALLS, bool TH, int CPT, bool AL, bool PER, int SRC>
 __global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs ae, const StepArgs ao, const int nze,
                                                                 double *__restrict__ partial) {
     // the even chunk's planes come first in dispatch order (alternating the
@@ -905,12 +915,12 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs a
     const bool even = (int)blockIdx.z < nze;
     const int zl = even ? (int)blockIdx.z : (int)blockIdx.z - nze;
     if (even) {
-        step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
+        step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, SRC, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
             ae, nullptr, grid_block(ae, CPT, ae.z0 + zl));
     } else {
         // the odd planes may wrap from Z - 1 to 0 (the last launch of a pair)
         const int z = ao.z0 + zl;
-        step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
+        step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, SRC, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
             ao, partial, grid_block(ao, CPT, z < ao.Z ? z : z - ao.Z));
     }
 }
@@ -982,6 +992,12 @@ __global__ void __launch_bounds__(256) k_mask(const uint8_t *__restrict__ ids, u
     nmask[c] = m;
 }
 
+// The FP32 kernels keep the per-launch source flag (measured: the TGV tick
+// pairs ran 2.8 % slower with the source compiled out, although they issue a
+// quarter fewer vector instructions; they are bound by memory, not issue).
+#ifndef BF_FP32_SRC_RUNTIME
+#define BF_FP32_SRC_RUNTIME 1
+#endif
 #ifndef BF_ROWBUF_16BIT_WALLS
 #define BF_ROWBUF_16BIT_WALLS 0  // build-time switch: row buffer also for 16-bit storage with walls
 #endif
@@ -1001,18 +1017,25 @@ hipError_t launch_step_st(const StepArgs &a_in, bool walls, bool mp, bool th, co
     // Closed grids run the kernels without neighbour checks: the plain kernel
     // on a periodic grid, and the wall kernels (plain walls, thermal walls,
     // Shan-Chen with thermal walls) where solid planes close the grid.
-#define BF_L(W, M, T, P)                                                                                   \
+#define BF_L2(W, M, T, P, S)                                                                               \
     do {                                                                                                   \
         if constexpr (!ODD && !M) {                                                                        \
             if (al) {                                                                                      \
-                hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, true, P>), sh.grid, sh.block, \
-                                   sh.lds_bytes, stream, a, partial);                                      \
+                hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, true, P, S>), sh.grid,    \
+                                   sh.block, sh.lds_bytes, stream, a, partial);                            \
                 return hipGetLastError();                                                                  \
             }                                                                                              \
         }                                                                                                  \
-        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, false, P>), sh.grid, sh.block, 0, \
-                           stream, a, partial);                                                            \
+        hipLaunchKernelGGL((k_step<VS, ST, ODD, W, M, T, (M || T) ? 1 : CPT, false, P, S>), sh.grid, sh.block, \
+                           0, stream, a, partial);                                                         \
         return hipGetLastError();                                                                          \
+    } while (0)
+#define BF_L(W, M, T, P)                                                                                   \
+    do {                                                                                                   \
+        if constexpr (M || T) BF_L2(W, M, T, P, kSrcOn);                                                   \
+        else if constexpr (ST == ST_FP32 && BF_FP32_SRC_RUNTIME) BF_L2(W, M, T, P, kSrcRuntime);           \
+        else if (a.source_on) BF_L2(W, M, T, P, kSrcOn);                                                   \
+        else BF_L2(W, M, T, P, kSrcOff);                                                                   \
     } while (0)
     const bool closed = a.closed != 0;
     if (walls) {
@@ -1029,6 +1052,7 @@ hipError_t launch_step_st(const StepArgs &a_in, bool walls, bool mp, bool th, co
     if (closed) BF_L(false, false, false, true);
     BF_L(false, false, false, false);
 #undef BF_L
+#undef BF_L2
 }
 
 template <int ST>
@@ -1043,11 +1067,18 @@ hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, int nze,
     const bool al = sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
     const dim3 grid(sh.grid.x, sh.grid.y, (unsigned)(nze + nzo));
     const size_t lds = al ? sh.lds_bytes : 0;
+#define BF_P2(W, T, A, P, S)                                                                               \
+    do {                                                                                                   \
+        hipLaunchKernelGGL((k_pair<VS, ST, W, T, T ? 1 : CPT, A, P, S>), grid, sh.block, lds, stream, ae, ao, \
+                           nze, partial);                                                                  \
+        return hipGetLastError();                                                                          \
+    } while (0)
 #define BF_P(W, T, A, P)                                                                                   \
     do {                                                                                                   \
-        hipLaunchKernelGGL((k_pair<VS, ST, W, T, T ? 1 : CPT, A, P>), grid, sh.block, lds, stream, ae, ao, nze, \
-                           partial);                                                                       \
-        return hipGetLastError();                                                                          \
+        if constexpr (T) BF_P2(W, T, A, P, kSrcOn);                                                        \
+        else if constexpr (ST == ST_FP32 && BF_FP32_SRC_RUNTIME) BF_P2(W, T, A, P, kSrcRuntime);           \
+        else if (ae.source_on) BF_P2(W, T, A, P, kSrcOn);                                                  \
+        else BF_P2(W, T, A, P, kSrcOff);                                                                   \
     } while (0)
     const bool closed = ae.closed != 0;
     if (walls) {
@@ -1065,6 +1096,7 @@ hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, int nze,
     if (al) BF_P(false, false, true, false);
     BF_P(false, false, false, false);
 #undef BF_P
+#undef BF_P2
 }
 
 template <int ST>
