@@ -262,17 +262,25 @@ def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
     ch = bfa.capi.Channel(res, 0.6, 0.05, velocity_set=vs)
     ch.set_porosity(porosity(*res))
     ch.forward(steps)                 # warm-up
-    J = ch.forward(steps)
-    fwd = ch.timing()[0]
-    while True:
-        try:
-            Jg, dg, du = ch.gradient(steps, checkpoint_every=k, trajectory_budget_bytes=budget)
-            break
-        except bfa.capi.BFError:
-            if budget == 0:
-                raise
-            budget //= 2              # device memory short of the estimate
-    t = ch.timing()
+    # both sides warm and repeated, the fastest run of each: the small D2Q9
+    # channel's launch-bound ticks vary by 10 % from run to run
+    fwd = float("inf")
+    for _ in range(3):
+        J = ch.forward(steps)
+        fwd = min(fwd, ch.timing()[0])
+    t = None
+    for _ in range(2):
+        while True:
+            try:
+                Jg, dg, du = ch.gradient(steps, checkpoint_every=k, trajectory_budget_bytes=budget)
+                break
+            except bfa.capi.BFError:
+                if budget == 0:
+                    raise
+                budget //= 2          # device memory short of the estimate
+        tt = ch.timing()
+        if t is None or tt[2] < t[2]:
+            t = tt
     ch.close()
     n = 1
     for r in res:
@@ -283,7 +291,8 @@ def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
                         f"U[0,1], {steps} ticks, checkpoint every {k}",
             "trajectory_budget_GB": round(budget / 1e9, 1), "resident_segments": resident,
             "segments": steps // k,
-            "J": J, "forward_ms": round(fwd, 3), "gradient_ms": round(t[2], 3), "recompute_ms": round(t[1], 3),
+            "J": J, "forward_ms": round(fwd, 3), "gradient_ms": round(t[2], 3),
+            "recompute_ms": round(t[1], 3), "recompute_beside_sweep": True,
             "ratio": round(t[2] / fwd, 3), "dJ_duin": du}, dg
 
 

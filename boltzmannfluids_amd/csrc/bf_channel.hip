@@ -420,6 +420,7 @@ struct bf_channel {
     int q = 0;
     int64_t N = 0, stride = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;         // segment recomputations, beside the adjoint sweep
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float *gamma = nullptr, *prof = nullptr;
     float *pp[2] = {nullptr, nullptr};     // forward ping-pong
@@ -470,14 +471,15 @@ void shape(const bf_channel *c, dim3 &grid, dim3 &block) {
     grid = dim3((X + bx - 1) / bx, (Y - 2 + by - 1) / by, Z);
 }
 
-hipError_t launch_tick(const bf_channel *c, const float *src, float *dst) {
+hipError_t launch_tick(const bf_channel *c, const float *src, float *dst, hipStream_t stream) {
     dim3 g, b;
     shape(c, g, b);
     const ChArgs a = args_of(c);
-    if (c->q == 9) hipLaunchKernelGGL(k_ch_step<D2Q9>, g, b, 0, c->stream, a, src, dst);
-    else hipLaunchKernelGGL(k_ch_step<D3Q19>, g, b, 0, c->stream, a, src, dst);
+    if (c->q == 9) hipLaunchKernelGGL(k_ch_step<D2Q9>, g, b, 0, stream, a, src, dst);
+    else hipLaunchKernelGGL(k_ch_step<D3Q19>, g, b, 0, stream, a, src, dst);
     return hipGetLastError();
 }
+hipError_t launch_tick(const bf_channel *c, const float *src, float *dst) { return launch_tick(c, src, dst, c->stream); }
 
 hipError_t launch_adjoint(const bf_channel *c, bool plain, const float *src, const float *nu, float *m_out, float *dg,
                           double *du_rows) {
@@ -521,8 +523,15 @@ hipError_t zero_f64(const bf_channel *c, double *p, int64_t n) {
 // Device buffer list freed together on every exit path.
 struct Scratch {
     std::vector<void *> ptrs;
+    std::vector<hipEvent_t> events;
     ~Scratch() {
         for (void *p : ptrs) (void)hipFree(p);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+    hipError_t event(hipEvent_t *e) {
+        hipError_t r = hipEventCreate(e);
+        if (r == hipSuccess) events.push_back(*e);
+        return r;
     }
     template <class T>
     hipError_t alloc(T **p, size_t bytes) {
@@ -567,7 +576,9 @@ bf_status bf_channel_create(const bf_channel_config *cfg, bf_channel **out) {
         return BF_ERR_DEVICE;
     };
     if (hipSetDevice(cfg->device) != hipSuccess) return bail("hipSetDevice");
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail("stream");
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess)
+        return bail("stream");
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail("event");
     const size_t sb = sizeof(float) * c->q * c->stride;
@@ -596,12 +607,14 @@ bf_status bf_channel_create(const bf_channel_config *cfg, bf_channel **out) {
 void bf_channel_destroy(bf_channel *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     for (void *p : {(void *)c->gamma, (void *)c->prof, (void *)c->pp[0], (void *)c->pp[1], (void *)c->rest,
                     (void *)c->scalar, (void *)c->moments})
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
 }
 
@@ -656,30 +669,37 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
     CH_HIP(c, hipSetDevice(c->cfg.device));
     const int64_t nseg = (T + K - 1) / K;
     const size_t sb = sizeof(float) * c->q * c->stride;
-    // segments the trajectory budget keeps resident (at least one buffer: the recompute target)
+    // segments the trajectory budget keeps resident (at least one)
     int64_t resident = std::max<int64_t>(1, acfg->trajectory_budget_bytes / (int64_t)(sb * K));
     resident = std::min(resident, nseg);
+    // Trajectory slots: one per resident segment, plus one when segments are
+    // recomputed, so that the recomputation of the next segment (second
+    // stream) runs beside the adjoint sweep of the current one.
+    const int64_t nrec = nseg - resident;
+    const int64_t nslots = resident + (nrec > 0 ? 1 : 0);
     Scratch sc;
     float *ckpt = nullptr, *traj = nullptr, *nu[2] = {nullptr, nullptr}, *dgseg = nullptr;
     double *dg64 = nullptr, *du_rows = nullptr;
     const int Y = c->cfg.resolution[1], Z = c->cfg.resolution[2];
     const int64_t per_seg = std::max<int64_t>(1, K - 1);
-    if (sc.alloc(&ckpt, sb * (size_t)nseg) != hipSuccess || sc.alloc(&traj, sb * (size_t)(per_seg * resident)) != hipSuccess ||
+    if (sc.alloc(&ckpt, sb * (size_t)nseg) != hipSuccess || sc.alloc(&traj, sb * (size_t)(per_seg * nslots)) != hipSuccess ||
         sc.alloc(&nu[0], sb) != hipSuccess || sc.alloc(&nu[1], sb) != hipSuccess ||
         sc.alloc(&dgseg, sizeof(float) * c->N) != hipSuccess || sc.alloc(&dg64, sizeof(double) * c->N) != hipSuccess ||
         sc.alloc(&du_rows, sizeof(double) * Y * Z) != hipSuccess)
-        return cfail(c, BF_ERR_OOM, "[LBM Error] bf_channel_gradient: cannot allocate %lld checkpoints + %lld resident segments",
-                     (long long)nseg, (long long)resident);
+        return cfail(c, BF_ERR_OOM, "[LBM Error] bf_channel_gradient: cannot allocate %lld checkpoints + %lld trajectory segments",
+                     (long long)nseg, (long long)nslots);
     const int64_t qs = (int64_t)c->q * c->stride;
     // Segment k covers ticks [kK, min(T, (k+1)K)).  State j of segment k is
     // s^{kK+j-1}, the state tick kK+j reads: j = 0 is checkpoint k, j >= 1 live
-    // in the segment's trajectory buffer (resident segments keep their own,
-    // the others are recomputed into buffer 0 once the resident ones are used).
+    // in the segment's trajectory slot (resident segments keep their own; a
+    // recomputed segment takes the extra slot or slot 0, whichever holds no
+    // segment still to be swept).
     auto resident_slot = [&](int64_t k) { return k >= nseg - resident ? k - (nseg - resident) : -1; };
+    std::vector<int64_t> seg_slot((size_t)nseg, -1);
+    for (int64_t k = nseg - resident; k < nseg; ++k) seg_slot[(size_t)k] = resident_slot(k);
     auto seg_state = [&](int64_t k, int64_t j) -> float * {
         if (j == 0) return ckpt + k * qs;
-        const int64_t r = resident_slot(k);
-        return traj + ((r >= 0 ? r : 0) * per_seg + (j - 1)) * qs;
+        return traj + (seg_slot[(size_t)k] * per_seg + (j - 1)) * qs;
     };
 
     // ---- forward with checkpoints (and the resident segments' trajectories) ----
@@ -715,18 +735,57 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
     }
     int cur = 0;          // nu[cur]: dJ/ds^T (the seed), then m of the tick after
     bool plain = true;    // only the first adjoint tick reads the seed slot by slot
-    double recompute_ms = 0.0;
+    // Recomputations run on the second stream, each timed by an event pair
+    // (read after the sweep: the host never waits between segments).  A
+    // recomputation into a slot waits for the sweep that last used the slot;
+    // the sweep of a recomputed segment waits for its recomputation.
+    std::vector<hipEvent_t> rec_ev((size_t)(2 * nrec)), slot_free((size_t)nslots);
+    for (auto &e : rec_ev)
+        if (sc.event(&e) != hipSuccess) return cfail(c, BF_ERR_DEVICE, "[LBM Error] bf_channel_gradient: cannot create events");
+    for (auto &e : slot_free)
+        if (sc.event(&e) != hipSuccess) return cfail(c, BF_ERR_DEVICE, "[LBM Error] bf_channel_gradient: cannot create events");
+    hipEvent_t fwd_done;
+    if (sc.event(&fwd_done) != hipSuccess) return cfail(c, BF_ERR_DEVICE, "[LBM Error] bf_channel_gradient: cannot create events");
+    std::vector<int64_t> owner((size_t)nslots, -1);   // segment held by the slot and not swept yet
+    for (int64_t k = nseg - resident; k < nseg; ++k) owner[(size_t)resident_slot(k)] = k;
+    std::vector<char> released((size_t)nslots, 0);     // slot_free recorded at least once
+    std::vector<int64_t> rec_of((size_t)nseg, -1);     // index of the segment's event pair
+    int64_t k_rec = nseg - resident - 1, n_rec = 0;     // next segment to recompute (descending)
+    CH_HIP(c, hipEventRecord(fwd_done, c->stream));
+    CH_HIP(c, hipStreamWaitEvent(c->stream2, fwd_done, 0));
+    auto try_recompute = [&]() -> hipError_t {
+        if (k_rec < 0) return hipSuccess;
+        const int64_t extra = resident;
+        int64_t sl = -1;
+        if (owner[(size_t)extra] < 0) sl = extra;
+        else if (owner[0] < 0) sl = 0;
+        if (sl < 0) return hipSuccess;                  // both slots hold segments still to be swept
+        hipError_t e;
+        if (released[(size_t)sl] && (e = hipStreamWaitEvent(c->stream2, slot_free[(size_t)sl], 0)) != hipSuccess) return e;
+        const int64_t k = k_rec--;
+        seg_slot[(size_t)k] = sl;
+        owner[(size_t)sl] = k;
+        rec_of[(size_t)k] = n_rec;
+        const int64_t len = std::min(K, T - k * K);
+        if ((e = hipEventRecord(rec_ev[(size_t)(2 * n_rec)], c->stream2)) != hipSuccess) return e;
+        for (int64_t j = 1; j < len; ++j)
+            if ((e = launch_tick(c, seg_state(k, j - 1), seg_state(k, j), c->stream2)) != hipSuccess) return e;
+        if ((e = hipEventRecord(rec_ev[(size_t)(2 * n_rec + 1)], c->stream2)) != hipSuccess) return e;
+        ++n_rec;
+        return hipSuccess;
+    };
+    CH_HIP(c, try_recompute());
     for (int64_t k = nseg - 1; k >= 0; --k) {
         const int64_t t0 = k * K, len = std::min(K, T - t0);
         if (resident_slot(k) < 0) {
-            // recompute the segment from its checkpoint into the recompute buffer
-            CH_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            for (int64_t j = 1; j < len; ++j) CH_HIP(c, launch_tick(c, seg_state(k, j - 1), seg_state(k, j)));
-            CH_HIP(c, hipEventRecord(c->ev[3], c->stream));
-            CH_HIP(c, hipEventSynchronize(c->ev[3]));
-            float ms = 0.0f;
-            CH_HIP(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-            recompute_ms += ms;
+            // every later segment is swept (enqueued), so a slot is free for k
+            while (rec_of[(size_t)k] < 0) {
+                const int64_t before = k_rec;
+                CH_HIP(c, try_recompute());
+                if (k_rec == before)
+                    return cfail(c, BF_ERR_STATE, "[LBM Error] bf_channel_gradient: no free trajectory slot (internal)");
+            }
+            CH_HIP(c, hipStreamWaitEvent(c->stream, rec_ev[(size_t)(2 * rec_of[(size_t)k] + 1)], 0));
         }
         for (int64_t j = len - 1; j >= 0; --j) {
             CH_HIP(c, launch_adjoint(c, plain, seg_state(k, j), nu[cur], nu[cur ^ 1], dgseg, du_rows));
@@ -735,6 +794,13 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
         }
         hipLaunchKernelGGL(k_fold, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, dgseg, dg64, c->N);
         CH_HIP(c, hipGetLastError());
+        const int64_t sl = seg_slot[(size_t)k];
+        if (sl >= 0) {
+            CH_HIP(c, hipEventRecord(slot_free[(size_t)sl], c->stream));
+            released[(size_t)sl] = 1;
+            owner[(size_t)sl] = -1;
+        }
+        CH_HIP(c, try_recompute());
     }
     CH_HIP(c, hipEventRecord(c->ev[3], c->stream));
     std::vector<double> dg(c->N), du((size_t)Y * Z);
@@ -743,6 +809,12 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
     CH_HIP(c, hipMemcpyAsync(dg.data(), dg64, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
     CH_HIP(c, hipMemcpyAsync(du.data(), du_rows, sizeof(double) * Y * Z, hipMemcpyDeviceToHost, c->stream));
     CH_HIP(c, hipStreamSynchronize(c->stream));
+    double recompute_ms = 0.0;
+    for (int64_t e = 0; e < n_rec; ++e) {
+        float ms = 0.0f;
+        CH_HIP(c, hipEventElapsedTime(&ms, rec_ev[(size_t)(2 * e)], rec_ev[(size_t)(2 * e + 1)]));
+        recompute_ms += ms;
+    }
     float fwd_ms = 0.0f, total_ms = 0.0f;
     CH_HIP(c, hipEventElapsedTime(&fwd_ms, c->ev[0], c->ev[1]));
     CH_HIP(c, hipEventElapsedTime(&total_ms, c->ev[0], c->ev[3]));
