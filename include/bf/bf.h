@@ -322,7 +322,9 @@ typedef struct bf_adjoint_config {
     int32_t checkpoint_every;         /* K: a checkpoint every K ticks */
     int64_t trajectory_budget_bytes;  /* device bytes for segments kept resident
                                          instead of recomputed (at least one
-                                         segment buffer is always used) */
+                                         segment buffer is always used, plus
+                                         one more while segments are
+                                         recomputed beside the adjoint sweep) */
 } bf_adjoint_config;
 
 bf_status bf_channel_create(const bf_channel_config *cfg, bf_channel **out);
@@ -339,7 +341,8 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *cfg, doubl
 /* Pre-collision (u_x, u_y, u_z, rho) per cell after the last run (4N floats). */
 bf_status bf_channel_read_moments(bf_channel *c, float *out, size_t bytes);
 /* Device times of the last run in ms: [0] forward (with checkpoints),
- * [1] recomputation, [2] whole gradient (0 for a forward-only run). */
+ * [1] recomputation (summed over segments; it runs on a second stream beside
+ * the adjoint sweep), [2] whole gradient (0 for a forward-only run). */
 bf_status bf_channel_timing(const bf_channel *c, double out[3]);
 
 /* --- checkpoint files ----------------------------------------------------
