@@ -211,8 +211,9 @@ def spinodal_thermal_config(bfa, n=192, steps=100, warmup=10):
         m1 = s.last_step_sums()[0]
         s.close()
         kms = ev / steps
-        # the synchronous pre-pass reads the lattices once more (27 + 7 slots)
-        extra = (27 + 7) * 4 + 4 if mode == "synchronous" else 0
+        # the synchronous pre-pass reads the main lattice once more (27 slots),
+        # the boundary id, and writes psi
+        extra = 27 * 4 + 1 + 4 if mode == "synchronous" else 0
         out[mode] = {"mlups": round(cells * steps / wall / 1e6, 1),
                      "roofline": roofline(2 * 27 * 4 + 2 * 7 * 4 + 8 + 1 + extra, cells, kms,
                                           load_traffic(f"step_d3q27_shanchen_thermal_{mode}_{n}")),

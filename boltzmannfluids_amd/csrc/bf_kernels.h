@@ -132,6 +132,13 @@ template <int VS>
 hipError_t launch_step(const StepArgs &a, int storage, bool odd, bool walls, bool multiphase,
                        bool thermal, const LaunchShape &shape, double *partial, hipStream_t stream);
 
+// Synchronous Shan-Chen density pre-pass of the coming tick: the tick's reads
+// of the main lattice, then psi of the floored density to a.psi_cur (no other
+// writes).  Same shape and parity as the tick it precedes.
+template <int VS>
+hipError_t launch_density(const StepArgs &a, int storage, bool odd, bool walls, const LaunchShape &shape,
+                          hipStream_t stream);
+
 // One launch of a chunked tick pair (non-multiphase kernels): the even tick
 // on `nze` planes from ae.z0 and the odd tick on `nzo` planes from ao.z0, in
 // one grid.  The two plane ranges must not overlap, and the even range must

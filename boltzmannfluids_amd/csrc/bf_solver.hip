@@ -242,6 +242,17 @@ int closed_domain(const bf_solver *s, const uint8_t *ids) {
     return 1;
 }
 
+hipError_t dispatch_density(const bf_solver *s, const StepArgs &a, bool odd) {
+    const bool walls = s->has_ids != 0;
+    switch (s->vs) {
+    case D2Q9: return launch_density<D2Q9>(a, s->storage, odd, walls, s->shape, s->stream);
+    case D3Q15: return launch_density<D3Q15>(a, s->storage, odd, walls, s->shape, s->stream);
+    case D3Q19: return launch_density<D3Q19>(a, s->storage, odd, walls, s->shape, s->stream);
+    case D3Q27: return launch_density<D3Q27>(a, s->storage, odd, walls, s->shape, s->stream);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t dispatch_mask(const bf_solver *s) {
     const int px = s->cfg.periodic[0] != 0, py = s->cfg.periodic[1] != 0, pz = s->cfg.periodic[2] != 0;
     switch (s->vs) {
@@ -810,12 +821,12 @@ static bf_status run_ticks(bf_solver *s, int64_t n, uint32_t flags) {
         if (last && (flags & BF_STEP_SAVE_MACRO)) { a.vd = s->vd; a.ft = s->ft; }
         double *partial = (last && (flags & BF_STEP_CONSERVATION)) ? s->partial : nullptr;
         if (s->cfg.multiphase && s->sc_sync) {
-            // density pre-pass: the same reads without writes of populations;
+            // density pre-pass: the tick's reads without writes of populations;
             // psi of this tick's density goes to psi[cur], which the tick then
             // reads as its neighbours' psi (and it writes its own to the other)
             StepArgs p = make_args(s, false);
             p.psi_cur = s->psi[s->cur];
-            const hipError_t e = dispatch_step(s, p, odd, nullptr);
+            const hipError_t e = dispatch_density(s, p, odd);
             if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] density pre-pass launch failed: %s", hipGetErrorString(e));
             a.psi_prev = s->psi[s->cur];
             a.psi_cur = s->psi[s->cur ^ 1];

@@ -223,7 +223,7 @@ __device__ inline BlockAt grid_block(const StepArgs &a, int cpt, int z) {
 // halves of k_pair and of k_list).  NTL / NTS: streaming (nontemporal)
 // population loads / stores.
 template <int VS, int ST, bool ODD, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, int SRC, bool NTL,
-          bool NTS>
+          bool NTS, bool DEN = false>
 __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict__ partial, const BlockAt blk) {
     const int z = blk.z;
     // PER: the grid is closed (every axis periodic, or solid boundary planes
@@ -247,6 +247,11 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     // direction computed and selected away: 7-8 vector instructions per
     // direction without a force.
     static_assert(SRC == kSrcOn || !(MP || TH), "the Shan-Chen and thermal kernels always have the source on");
+    // DEN: the synchronous Shan-Chen density pre-pass.  The tick's reads (the
+    // same addresses, bounce-back, moving-wall and open-boundary rules, so the
+    // same density bit for bit), then psi of the floored density is written
+    // and the cell is done: no tile, force, collision or thermal lattice.
+    static_assert(!DEN || (MP && !TH && !AL && CPT == 1), "the density pre-pass is a multiphase read pass");
     extern __shared__ float lds_rows[];
     using L = Lattice<VS>;
     using S = Store<ST, NTL, NTS>;
@@ -336,8 +341,8 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
 #define NB(k, dx, dy, dz) ((dx) == 1 ? ROW(k, dy, dz) : POP(xs[k][dx] + ry[dy] + pz[dz]))
 
     // Shan-Chen: stage psi of the previous tick for the block tile + halo.
-    __shared__ float tile[MP ? kTileX * kTileY * 3 : 1];
-    if constexpr (MP) {
+    __shared__ float tile[(MP && !DEN) ? kTileX * kTileY * 3 : 1];
+    if constexpr (MP && !DEN) {
         const int tid = ty * kMpBX + tx;
         const int nplanes = D3 ? 3 : 1;
         const int x0 = blk.xb - 1, y0 = blk.yb - 1;
@@ -359,7 +364,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
 #ifndef BF_SC_SEPARABLE
 #define BF_SC_SEPARABLE 1  // build-time switch of the separable D3Q27 pseudopotential gradient
 #endif
-    constexpr bool kSep = BF_SC_SEPARABLE && MP && VS == D3Q27;
+    constexpr bool kSep = BF_SC_SEPARABLE && MP && !DEN && VS == D3Q27;
     constexpr int kSepN = kTileX * kMpBY;
     constexpr float k23 = float(2.0 / 3.0), k16 = float(1.0 / 6.0);
     __shared__ float sep[kSep ? 3 * kSepN : 1];
@@ -648,6 +653,10 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
             rho = low ? 0.01f : rho;
             d = low ? -0.99f : d;
         }
+        if constexpr (DEN) {
+            if (a.write_psi) a.psi_cur[self[k]] = 1.0f - expf(-rho);  // as psi_self below
+            continue;
+        }
 
         // force (collide.comp:707-720)
         float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
@@ -883,6 +892,13 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY k_step(const StepArgs a, dou
                                                                       grid_block(a, CPT, a.z0 + (int)blockIdx.z));
 }
 
+// Synchronous Shan-Chen: psi of this tick's densities, before the tick.
+template <int VS, int ST, bool ODD, bool WALLS, bool PER>
+__global__ void __launch_bounds__(512) k_density(const StepArgs a) {
+    step_body<VS, ST, ODD, WALLS, true, false, 1, false, PER, kSrcOn, true, true, true>(
+        a, nullptr, grid_block(a, 1, a.z0 + (int)blockIdx.z));
+}
+
 // A launch of a chunked tick pair: the blocks of the first `nze` planes run
 // the even tick on planes [ae.z0, ae.z0 + nze), the others the odd tick on
 // planes from ao.z0.  The host schedules chunk j's even tick with chunk j-2's
@@ -1099,6 +1115,23 @@ hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, int nze,
 #undef BF_P2
 }
 
+template <int ST, bool ODD>
+hipError_t launch_density_st(const StepArgs &a_in, bool walls, const LaunchShape &sh, hipStream_t stream) {
+    constexpr int VS = BF_VS;
+    StepArgs a = a_in;
+    a.bdx = (int)sh.block.x;
+    a.bdy = (int)sh.block.y;
+    const bool closed = a.closed != 0;
+#define BF_D(W, P)                                                                                         \
+    do {                                                                                                   \
+        hipLaunchKernelGGL((k_density<VS, ST, ODD, W, P>), sh.grid, sh.block, 0, stream, a);               \
+        return hipGetLastError();                                                                          \
+    } while (0)
+    if (walls) { if (closed) BF_D(true, true); BF_D(true, false); }
+    BF_D(false, false);
+#undef BF_D
+}
+
 template <int ST>
 hipError_t launch_init_st(const InitArgs &a, bool th, hipStream_t stream) {
     const dim3 block(256), grid((unsigned)((a.n + 255) / 256));
@@ -1119,6 +1152,20 @@ hipError_t launch_step<BF_VS>(const StepArgs &a, int storage, bool odd, bool wal
                              : launch_step_st<ST_FP16, false>(a, walls, mp, th, sh, partial, stream);
     case ST_BF16: return odd ? launch_step_st<ST_BF16, true>(a, walls, mp, th, sh, partial, stream)
                              : launch_step_st<ST_BF16, false>(a, walls, mp, th, sh, partial, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <>
+hipError_t launch_density<BF_VS>(const StepArgs &a, int storage, bool odd, bool walls, const LaunchShape &sh,
+                                 hipStream_t stream) {
+    switch (storage) {
+    case ST_FP32: return odd ? launch_density_st<ST_FP32, true>(a, walls, sh, stream)
+                             : launch_density_st<ST_FP32, false>(a, walls, sh, stream);
+    case ST_FP16: return odd ? launch_density_st<ST_FP16, true>(a, walls, sh, stream)
+                             : launch_density_st<ST_FP16, false>(a, walls, sh, stream);
+    case ST_BF16: return odd ? launch_density_st<ST_BF16, true>(a, walls, sh, stream)
+                             : launch_density_st<ST_BF16, false>(a, walls, sh, stream);
     }
     return hipErrorInvalidValue;
 }
