@@ -199,7 +199,10 @@ constexpr uint32_t kNoValue = 0x7fa5a5a5u;
 #ifndef BF_WAVES_PER_EU
 #define BF_WAVES_PER_EU 8
 #endif
-#define BF_OCCUPANCY __attribute__((amdgpu_waves_per_eu((MP || TH) ? 1 : BF_WAVES_PER_EU)))
+#ifndef BF_WAVES_PER_EU_MPTH
+#define BF_WAVES_PER_EU_MPTH 1  // 1: the compiler's choice
+#endif
+#define BF_OCCUPANCY __attribute__((amdgpu_waves_per_eu((MP || TH) ? BF_WAVES_PER_EU_MPTH : BF_WAVES_PER_EU)))
 
 // v modulo n in [0, n).  Tile coordinates are inside the grid except in the
 // halo at the grid's edges, so the integer division only runs on those lanes.
