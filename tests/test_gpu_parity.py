@@ -359,10 +359,13 @@ def test_shan_chen_sets_and_storage(bfa, orc, vs, storage, walls):
         assert dpsi <= (1e-3 if storage == "fp16" else 1e-2)
 
 
-@pytest.mark.parametrize("vs,shape,periodic", [("D3Q27", (16, 16, 16), (1, 1, 1)), ("D2Q9", (70, 20, 1), (1, 0, 1))])
+@pytest.mark.parametrize("vs,shape,periodic", [("D3Q27", (16, 16, 16), (1, 1, 1)), ("D2Q9", (70, 20, 1), (1, 0, 1)),
+                                               ("D3Q19", (18, 14, 10), (0, 0, 1))])
 def test_shan_chen_synchronous(bfa, orc, vs, shape, periodic):
     """Synchronous pseudopotential (density pre-pass per tick) against the
-    oracle's; it must also differ from the lagged run of the same case."""
+    oracle's; it must also differ from the lagged run of the same case.  The
+    pre-pass runs on a periodic grid, a grid closed by walls and one with
+    walls and open x edges (0.01 reads)."""
     N = shape[0] * shape[1] * shape[2]
     rng = np.random.default_rng(SEED + 11)
     rho = (0.3 + 0.05 * rng.random(N)).astype(np.float32)
@@ -370,7 +373,7 @@ def test_shan_chen_synchronous(bfa, orc, vs, shape, periodic):
     init = dict(rho=rho, u=np.zeros((N, 3), np.float32))
     objects = []
     if periodic[1] == 0:
-        init["boundary_id"] = channel_ids(shape[0], shape[1])
+        init["boundary_id"] = channel_ids(shape[0], shape[1], shape[2])
         objects = [(1, dict(effective_density=1.5))]
     params = [("interaction_strength", -5.0), ("shan_chen_synchronous", 1.0)]
     g, o = run_pair(bfa, orc, shape, 0.9, 31, velocity_set=vs, multiphase=True, periodic=periodic,
