@@ -719,21 +719,24 @@ bf_status bf_get_param(const bf_solver *s, int32_t key, float *v, int32_t n) {
 namespace {
 
 // Chunked tick pairs.  An even tick followed by an odd tick can run chunk by
-// chunk of z-planes: the odd tick of a chunk needs the even tick of that chunk
-// and of its two neighbour chunks (the even tick writes into the planes next
-// to its own), so chunk j-2's odd tick can share a launch with chunk j's even
-// tick (run_pair).  The lines the even tick wrote are then still in the
+// chunk of z-planes: the odd tick of a plane needs the even tick of that plane
+// and of its two neighbours (the even tick writes into the planes next to its
+// own), so the odd tick of the planes up to two before chunk j can share a
+// launch with chunk j's even tick (run_pair).  The lines the even tick wrote
+// are then still in the
 // 256 MB last-level cache when the odd tick reads them.  Measured on the
-// MI355X (D3Q19, device time per tick, interleaved on one solver, ranges
-// over boxes): the 384^3 porous duct 1.74-1.75 -> 1.575-1.586 ms (FP32,
-// chunks of 8 planes; 6 planes 1.61-1.62, 10 planes 1.70) and 1.08 ->
-// 1.03-1.04 ms (FP16, 14-16 planes); the fully periodic 256^3 box
-// 406-407 -> 390-393 us in FP32 (18 planes; 16 planes 387-399, 12 planes 413,
-// 24 planes 418), but 208.7 -> 213-234 us in FP16 (16-48 planes), whose even
-// tick carries the most instructions per byte.  So the automatic mode pairs
-// the ticks of FP32 lattices, and of any lattice with walls, above
-// kPairMinBytes, in chunks of about kPairChunkBytes.
-constexpr double kPairChunkBytes = 90e6;
+// MI355X (D3Q19, device time per tick, interleaved on one solver): with the
+// odd planes one plane behind the even chunk, the 256^3 periodic box in FP32
+// 378-381 us per tick with chunks of 24 planes (119 MB; 20 planes 381.5, 28
+// planes 376.5-379.8, whole-grid ticks 407-413), the 384^3 periodic box
+// 1294-1297 us (10-12 planes; 16 planes 1415), the 384^3 porous duct
+// 1472-1474 us in FP32 (10 planes; 12 planes 1482, 14 planes 1546; whole-grid
+// ticks 1733-1742) and 862-871 us in FP16 (20-24 planes; 28 planes 877-880,
+// whole-grid ticks 1036-1041).  With the odd chunk two chunks behind the
+// even one the best chunks held 80-90 MB (256^3: 384 us per tick).  The
+// automatic mode pairs the ticks of FP32 lattices, and of any lattice with
+// walls, above kPairMinBytes, in chunks of about kPairChunkBytes.
+constexpr double kPairChunkBytes = 120e6;
 constexpr double kPairMinBytes = 384e6;
 
 // planes per chunk, or 0 for whole-grid ticks
@@ -765,15 +768,12 @@ hipError_t dispatch_pair(const bf_solver *s, const StepArgs &ae, const StepArgs 
 }
 
 // An even tick and the odd tick after it, in chunks of zc planes (m >= 4
-// chunks), on the handle's stream:
-//   E0 E1 E2 | E3 + O1 | ... | E(m-1) + O(m-3) | O(m-2) O(m-1) O0
-// The odd tick of chunk l needs the even ticks of chunks l-1, l and l+1 (the
-// even tick writes into the planes beside its own; chunks 0 and m-1 are
-// neighbours on a periodic z axis), all of them in earlier launches; within a
-// launch the even chunk j and the odd chunk j-2 are at least one plane apart.
-// The first launch holds three even chunks and the last three odd chunks
-// (their planes wrap from Z-1 to 0): even ticks of different cells, or odd
-// ticks, never touch one another's slots.  `flags` apply to the odd tick.
+// chunks), on the handle's stream: m + 1 launches
+//   E0 | E1 + O[1, lo(1) - 1) | ... | E(m-1) + O[.., lo(m-1) - 1) | O[lo(m-1) - 1, Z) O0
+// Within a launch the odd planes end two planes before the even chunk: even
+// ticks of different cells, or odd ticks, never touch one another's slots.
+// The odd planes read what the even chunk of the previous launch wrote, which
+// the last-level cache still holds.  `flags` apply to the odd tick.
 bf_status run_pair(bf_solver *s, int zc, uint32_t flags) {
     const int Z = s->Z, m = (Z + zc - 1) / zc;
     StepArgs ae = make_args(s, true), ao = make_args(s, true);
@@ -790,9 +790,20 @@ bf_status run_pair(bf_solver *s, int zc, uint32_t flags) {
         if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] tick-pair launch failed: %s", hipGetErrorString(e));
         return BF_OK;
     };
-    bf_status st = launch(0, hi(2), 0, 0);
-    for (int j = 3; j < m && st == BF_OK; ++j) st = launch(lo(j), hi(j) - lo(j), lo(j - 2), hi(j - 2) - lo(j - 2));
-    if (st == BF_OK) st = launch(0, 0, lo(m - 2), Z - lo(m - 2) + hi(0));
+    // The odd tick one plane behind the even tick: launch j runs the even
+    // tick on chunk j and the odd tick on every plane p in 1 .. lo(j) - 2 not
+    // yet done (plane p needs the even ticks of p - 1, p and p + 1, all in
+    // earlier launches; this launch's even chunk reads and writes from plane
+    // lo(j) - 1 on).  Plane 0 and the planes from lo(m - 1) - 1 come last: on
+    // a periodic z axis the last chunk's even tick writes into plane 0.
+    bf_status st = launch(0, hi(0), 0, 0);
+    int zo = 1;
+    for (int j = 1; j < m && st == BF_OK; ++j) {
+        const int nzo = std::max(0, lo(j) - 1 - zo);
+        st = launch(lo(j), hi(j) - lo(j), zo, nzo);
+        zo += nzo;
+    }
+    if (st == BF_OK) st = launch(0, 0, zo, Z - zo + 1);
     if (st != BF_OK) return st;
     if (partial) {
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);

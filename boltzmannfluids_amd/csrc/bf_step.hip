@@ -904,13 +904,13 @@ __global__ void __launch_bounds__(512) k_density(const StepArgs a) {
 
 // A launch of a chunked tick pair: the blocks of the first `nze` planes run
 // the even tick on planes [ae.z0, ae.z0 + nze), the others the odd tick on
-// planes from ao.z0.  The host schedules chunk j's even tick with chunk j-2's
-// odd tick, so the two halves touch disjoint planes (the even tick of a plane
-// reads and writes its neighbour planes z +- 1; the odd tick stays at its
-// cells).  The even tick's accesses are ordinary loads and stores, whose
-// lines the 256 MB last-level cache keeps; the odd tick of that chunk, two
-// launches later, finds most of them there and streams its own traffic with
-// nontemporal hints.  (Which hints the two halves use measured within noise
+// planes from ao.z0.  The host schedules chunk j's even tick with the odd
+// tick of the planes that end two planes before the chunk, so the two halves
+// touch disjoint planes (the even tick of a plane reads and writes its
+// neighbour planes z +- 1; the odd tick stays at its cells).  The even tick's
+// accesses are ordinary loads and stores, whose lines the 256 MB last-level
+// cache keeps; the odd tick of those planes, one launch later, finds most of
+// them there and streams its own traffic with nontemporal hints.  (Which hints the two halves use measured within noise
 // of each other on the tick kernels; they decide the rate of the bare access
 // pattern.)
 #ifndef BF_PAIR_EVEN_NT_LOAD
