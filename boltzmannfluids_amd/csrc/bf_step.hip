@@ -247,7 +247,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     // SRC: the Guo source (a force, Shan-Chen or a thermal lattice) is off
     // (kSrcOff), on (kSrcOn), or decided per launch (kSrcRuntime).  The
     // launch's flag tested per direction is compiled as the source of every
-    // direction computed and selected away: 7-8 vector instructions per
+    // direction computed and selected away: 3-4 vector instructions per
     // direction without a force.
     static_assert(SRC == kSrcOn || !(MP || TH), "the Shan-Chen and thermal kernels always have the source on");
     // DEN: the synchronous Shan-Chen density pre-pass.  The tick's reads (the
@@ -500,6 +500,18 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     // term can be non-zero: a moving wall or a thermal wall)
 #define WALL(k, j) a.objects[a.ids[xs[k][1 + L::c[j][0]] + ry[1 + L::c[j][1]] + pz[1 + L::c[j][2]]]]
     // 1c
+    // Every load of 1b stays ahead of every use in 1c where that measured
+    // faster (device time per tick, interleaved builds): the FP32 even tick
+    // without walls (left to itself the scheduler splits its loads into
+    // batches with waits between them: tick pairs 5 % slower), and the 16-bit
+    // wall kernels (porous duct 1-1.5 % faster).  Not for the FP32 wall
+    // kernels (+2 %) or the Shan-Chen kernels (+1 %).
+#ifndef BF_SCHED_SPLIT
+#define BF_SCHED_SPLIT 1  // build-time switch of the scheduling barrier
+#endif
+    constexpr bool kSchedSplit = BF_SCHED_SPLIT && !MP && !TH &&
+                                 ((ST == ST_FP32 && AL && !WALLS) || (ST != ST_FP32 && WALLS));
+    if constexpr (kSchedSplit) __builtin_amdgcn_sched_barrier(0);
     if constexpr (AL) {
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
@@ -725,12 +737,18 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
             const float fu = __builtin_fmaf(Fz, uz, __builtin_fmaf(Fy, uy, Fx * ux));
             const float c1 = -1.5f * uu;
             const float inv_tau = a.inv_tau;
-            const float pre = a.guo_pre;
+            const float pre3 = 3.0f * a.guo_pre;
+            // c_i . u and c_i . F of the last odd direction: the even direction
+            // after it is its opposite and takes the negated products
+            float cu_odd = 0.0f, cf_odd = 0.0f;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 const float w = L::w[i];
                 const int o = opposite(i);
-                const float cu = (i == 0) ? 0.0f : dotc<BF_VS>(i, ux, uy, uz);
+                static_assert(opposite(2) == 1 && opposite(Q - 1) == Q - 2, "opposite pairs are (2j - 1, 2j)");
+                float cu = 0.0f;
+                if (i != 0 && (i & 1)) { cu = dotc<BF_VS>(i, ux, uy, uz); cu_odd = cu; }
+                else if (i != 0) cu = -cu_odd;
                 // h_eq = w rho (1 + 3 cu + 9/2 cu^2 - 3/2 u^2) - w
                 //      = (w rho)(cu (3 + 9/2 cu) - 3/2 u^2) + w d,
                 // the rest population's mass part with w_0 + rest_fix, rounded
@@ -742,8 +760,12 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
                 // threshold (a select, not a per-direction branch)
                 v = __builtin_fmaf(e - fi[k][i], inv_tau, fi[k][i]);
                 if (SRC == kSrcOn || (SRC == kSrcRuntime && a.source_on)) {
-                    const float cf = (i == 0) ? 0.0f : dotc<BF_VS>(i, Fx, Fy, Fz);
-                    v += (pre * w) * (__builtin_fmaf(9.0f * cf, cu, 3.0f * cf) - 3.0f * fu);
+                    float cf = 0.0f;
+                    if (i != 0 && (i & 1)) { cf = dotc<BF_VS>(i, Fx, Fy, Fz); cf_odd = cf; }
+                    else if (i != 0) cf = -cf_odd;
+                    // (1 - 1/(2 tau)) w (3 cF + 9 cF cu - 3 F.u), as
+                    // 3 pre w (cF (3 cu + 1) - F.u): three fused multiply-adds
+                    v = __builtin_fmaf(pre3 * w, __builtin_fmaf(cf, __builtin_fmaf(3.0f, cu, 1.0f), -fu), v);
                 }
                 v = ext ? e : v;
                 if (ODD) {

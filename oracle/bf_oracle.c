@@ -484,20 +484,32 @@ static void tick(orc_solver *s, int update, int save, int want_sums) {
             REAL c1 = (REAL)-1.5 * uu;
             REAL inv_tau = (REAL)1 / s->tau;
             REAL pre = (REAL)1 - (REAL)1 / ((REAL)2 * s->tau);
+            REAL pre3 = (REAL)3 * pre;
             int src_on = s->forcing || s->multiphase || s->thermal;
+            /* c.u and c.F of the last odd direction: the even direction after
+             * it is its opposite (VelocitySet.cpp:53-141 order) and takes the
+             * negated products */
+            REAL cu_odd = 0, cf_odd = 0;
             for (int i = 0; i < s->q; ++i) {
                 const int *c = s->T[i].c;
                 REAL w = (REAL)s->T[i].w;
-                REAL cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
+                REAL cu, cf;
+                if (i == 0 || (i & 1)) {
+                    cu = (REAL)c[0] * u[0] + (REAL)c[1] * u[1] + (REAL)c[2] * u[2];
+                    cf = (REAL)c[0] * F[0] + (REAL)c[1] * F[1] + (REAL)c[2] * F[2];
+                    cu_odd = cu; cf_odd = cf;
+                } else {
+                    cu = -cu_odd; cf = -cf_odd;
+                }
                 REAL e = heq(w, rho, d, cu, c1, i == 0 ? s->fix : (REAL)0);
                 REAL v;
                 if (ext) v = e;
                 else {
                     v = FMA(e - f[i], inv_tau, f[i]);   /* h - (h - heq)/tau */
                     if (src_on) {
-                        REAL cf = (REAL)c[0] * F[0] + (REAL)c[1] * F[1] + (REAL)c[2] * F[2];
-                        /* Guo: (1 - 1/(2 tau)) w (3 cF + 9 cF cu - 3 F.u) */
-                        v += (pre * w) * (FMA((REAL)9 * cf, cu, (REAL)3 * cf) - (REAL)3 * fu);
+                        /* Guo: (1 - 1/(2 tau)) w (3 cF + 9 cF cu - 3 F.u),
+                         * grouped as 3 pre w (cF (3 cu + 1) - F.u) */
+                        v = FMA(pre3 * w, FMA(cf, FMA((REAL)3, cu, (REAL)1), -fu), v);
                     }
                 }
                 write_pop(s, x, y, z, i, odd, v);
