@@ -35,11 +35,10 @@ constexpr int kMpBX = 64;   // multiphase tile: 64 x 4 cells, one z-plane
 constexpr int kMpBY = 4;
 constexpr int kTileX = kMpBX + 2, kTileY = kMpBY + 2;
 
-__device__ inline uint16_t bf16_rne(float v) {
-    uint32_t b = __float_as_uint(v);
-    b += 0x7fffu + ((b >> 16) & 1u);
-    return (uint16_t)(b >> 16);
-}
+// FP32 -> BF16, round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
+// (the integer form, add 0x7fff + lsb and keep the high half, is two
+// instructions per store)
+__device__ inline uint16_t bf16_rne(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 
 // Streaming hints for the population traffic: every slot is read once and
 // written once per tick, so the accesses may bypass cache allocation.
