@@ -215,6 +215,48 @@ def test_momentum_gain_constant_force(orc, name):
     assert abs(s1[0] - s0[0]) < 1e-12 * s0[0]
 
 
+@pytest.mark.parametrize("name", SETS)
+def test_guo_source_single_tick_closed_form(orc, name):
+    """One tick of a uniform forced state against the textbook BGK + Guo update
+    evaluated here in float64: f' = f - (f - f_eq(rho, u))/tau + S with u = m/rho
+    + F/(2 rho) and S_i = (1 - 1/(2 tau)) w_i [3 (c_i - u).F + 9 (c_i.u)(c_i.F)]
+    (Guo, Zheng & Shi, PRE 65:046308, 2002).  Every term of S is large enough
+    here (F.u ~ 1e-4) that a dropped or mis-signed one fails by orders of
+    magnitude; the momentum test above pins only the 3 c.F term."""
+    Z = 1 if name == "D2Q9" else 2
+    tau = 0.7
+    o = orc.Oracle((3, 2, Z), tau, velocity_set=name)
+    n = o.cells
+    tab = orc.velocity_table(name)
+    c, w = tab[:, :3], tab[:, 3]
+    q = len(w)
+    rho0 = np.float64(np.float32(1.03))
+    u0 = np.float32([0.04, -0.03, 0.02 if Z > 1 else 0.0]).astype(np.float64)
+    F = np.float32([2e-3, -1e-3, 1.5e-3 if Z > 1 else 0.0]).astype(np.float64)
+    o.init_fields(np.full(n, rho0), np.tile(u0, (n, 1)), force=np.tile(F, (n, 1)))
+
+    def feq(rho, u):
+        cu = c @ u
+        return w * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * (u @ u))
+
+    f = feq(rho0, u0)
+    rho = f.sum()
+    u = (f @ c) / rho + F / (2 * rho)
+    cu, cf = c @ u, c @ F
+    S = (1 - 1 / (2 * tau)) * w * (3 * (cf - u @ F) + 9 * cu * cf)
+    v = f - (f - feq(rho, u)) / tau + S
+    o.step(1)
+    raw = o.read("populations_raw")
+    # after the even tick slot j holds the post-collision population of the
+    # opposite direction (in a uniform state streaming moves nothing else)
+    opp = [0] + [i + 1 if i % 2 else i - 1 for i in range(1, q)]
+    np.testing.assert_allclose(raw, np.tile(v[opp][:, None], (1, n)), rtol=0, atol=1e-13)
+    # the source moves no mass and adds (1 - 1/(2 tau)) (F u + u F) to the stress
+    assert abs(S.sum()) < 1e-15
+    stress = np.einsum("i,ia,ib->ab", S, c, c)
+    np.testing.assert_allclose(stress, (1 - 1 / (2 * tau)) * (np.outer(F, u) + np.outer(u, F)), atol=1e-15)
+
+
 def test_taylor_green_decay_matches_viscosity(orc):
     """Low-Re 3-D Taylor-Green: KE(t) ~ KE0 exp(-6 nu k^2 t), nu = (tau - 1/2)/3."""
     n, u0, tau = 24, 0.005, 0.8
