@@ -734,8 +734,13 @@ namespace {
 // ticks 1733-1742) and 862-871 us in FP16 (20-24 planes; 28 planes 877-880,
 // whole-grid ticks 1036-1041).  With the odd chunk two chunks behind the
 // even one the best chunks held 80-90 MB (256^3: 384 us per tick).  The
-// automatic mode pairs the ticks of FP32 lattices, and of any lattice with
-// walls, above kPairMinBytes, in chunks of about kPairChunkBytes.
+// periodic 16-bit boxes gain once their even tick issues few enough
+// instructions: 256^3 FP16 193.6-196.6 us per tick with 48-plane chunks
+// (120 MB; 32 planes 197.6, 64 planes 194.8, whole-grid ticks 199.4-200.0),
+// BF16 195.6-196.2 against 199.8; 384^3 FP16 679.1-679.8 us with 21 planes
+// (28 planes 675.1-675.3, whole-grid ticks 703.9-706.2).  The automatic mode
+// pairs the ticks of every lattice above kPairMinBytes, in chunks of about
+// kPairChunkBytes.
 constexpr double kPairChunkBytes = 120e6;
 constexpr double kPairMinBytes = 384e6;
 
@@ -744,7 +749,6 @@ int pair_chunk(const bf_solver *s) {
     if (s->cfg.multiphase || s->pair_planes < 0) return 0;
     int zc = s->pair_planes;
     if (zc == 0) {
-        if (!s->has_ids && s->elem != 4) return 0;
         const double plane = (double)s->X * s->Y * ((double)s->q * s->elem + (s->thermal ? 4.0 * s->qt : 0.0));
         if (plane * s->Z < kPairMinBytes) return 0;
         zc = std::max(1, (int)std::lround(kPairChunkBytes / plane));

@@ -409,6 +409,18 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
 
     double s_rho = 0.0, s_mx = 0.0, s_my = 0.0, s_mz = 0.0, s_t = 0.0;
 
+    // Plain periodic kernels: every lane loads in range (lanes past the grid
+    // take its last row or column), so the reads, the row-buffer exchange and
+    // the conversions run on every lane and only the stores test `active`.
+    // Behind a branch the skipped path would need its registers set (zeros,
+    // the open-boundary constants, copies): about 40 instructions per tick.
+    // A lane past the grid in x writes the row-buffer entry of the grid's
+    // last column, the value that column's own cell writes there.
+#ifndef BF_ALL_LANES
+#define BF_ALL_LANES 1
+#endif
+    constexpr bool kAllLanes = BF_ALL_LANES && PER && !WALLS && !MP && !TH;
+
     // ---- phase 1: AA reads (collide.comp:244-305) ----
     // 1b issues every population load of the tick (addresses from the
     // neighbour mask), then the ids of solid neighbours where wall terms need
@@ -514,7 +526,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     if constexpr (AL) {
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
-            if (!in_bounds[k]) continue;
+            if (!kAllLanes && !in_bounds[k]) continue;
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;
@@ -524,7 +536,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
     }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        if (!active[k]) continue;
+        if (!kAllLanes && !active[k]) continue;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const float w = L::w[i];
@@ -566,7 +578,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
         // pull the x-shifted values from the row buffer
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
-            if (!active[k]) continue;
+            if (!kAllLanes && !active[k]) continue;
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 if (L::c[i][0] == 0) continue;

@@ -110,11 +110,11 @@ typedef enum bf_param {
     BF_PARAM_SHAN_CHEN_SYNCHRONOUS = 9,
     /* 1 float (default 0): how bf_step runs an even tick and the odd tick
      * after it.  -1 = each tick over the whole grid; k > 0 = the pair chunk
-     * by chunk of k z-planes, chunk j's even tick in one launch with chunk
-     * j-2's odd tick, so that the odd tick reads what the even tick wrote
-     * from the last-level cache; 0 = chunks of about 90 MB of lattice for
-     * FP32 lattices and for grids with walls whose lattice exceeds 384 MB
-     * (where it was measured faster), else whole-grid ticks.  Results are
+     * by chunk of k z-planes, chunk j's even tick in one launch with the odd
+     * tick of the planes up to two before it, so that the odd tick reads
+     * what the previous launch's even tick wrote from the last-level cache;
+     * 0 = chunks of about 120 MB of lattice for every lattice that exceeds
+     * 384 MB (where it was measured faster), else whole-grid ticks.  Results are
      * identical in every mode (each cell sees the same operations); the
      * Shan-Chen solver always runs whole-grid ticks, and so does a grid of
      * fewer than four chunks.  No counterpart in the reference. */

@@ -145,9 +145,9 @@ def test_pairs_equal_whole_grid_ticks_porous_96(bfa, storage):
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 def test_automatic_mode_on_the_periodic_tgv_256(bfa, storage):
-    """The automatic mode pairs the ticks of the 256^3 FP32 box (1.3 GB, no
-    walls) and not of the FP16 one; either way the result is bit-identical to
-    whole-grid ticks (6 ticks, saved fields on the last)."""
+    """The automatic mode pairs the ticks of the 256^3 box (1.3 GB in FP32,
+    0.64 GB in FP16; no walls), bit-identical to whole-grid ticks (6 ticks,
+    saved fields on the last)."""
     from boltzmannfluids_amd.scenarios import taylor_green
     n = 256
     say(f"case: automatic tick pairs TGV {n}^3 {storage}")
