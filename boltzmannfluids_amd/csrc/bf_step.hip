@@ -456,7 +456,15 @@ __device__ __forceinline__ void step_body(const StepArgs &a, double *__restrict_
                 raw[k][i] = S::raw(SLOT(i), idx);
             }
         }
-        const bool act = kEvery || active[k];
+        // Closed 16-bit wall kernels: every lane's addresses lie in the grid
+        // (a solid cell's neighbours included), so solid cells load their own
+        // neighbourhood, lines their fluid neighbours read as well, and no
+        // select redirects them to cell 0.
+#ifndef BF_SOLID_LOADS
+#define BF_SOLID_LOADS 1
+#endif
+        constexpr bool kAnyAddr = BF_SOLID_LOADS && PER && WALLS && ST != ST_FP32 && !MP && !TH;
+        const bool act = kEvery || kAnyAddr || active[k];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int o = opposite(i);
