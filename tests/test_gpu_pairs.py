@@ -102,17 +102,18 @@ def test_pairs_walls_and_open_faces_match_oracle(bfa, orc, storage, periodic):
         check(g, o, storage, "D3Q19", orc)
 
 
-@pytest.mark.parametrize("vs", ["D3Q19", "D3Q27"])
-def test_pairs_thermal_match_oracle(bfa, orc, vs):
-    """The thermal lattice (D3Q7, in place like the main lattice) in pairs:
-    hot and cold walls with buoyancy."""
-    say(f"case: pairs thermal {vs}")
-    g, o = make_pair(bfa, orc, (12, 10, 10), vs, "fp32", 2, periodic=(1, 0, 1), thermal=True)
+@pytest.mark.parametrize("vs,storage", [("D3Q19", "fp32"), ("D3Q27", "fp32"), ("D3Q19", "fp16")])
+def test_pairs_thermal_match_oracle(bfa, orc, vs, storage):
+    """The thermal lattice (D3Q7, FP32, in place like the main lattice) in
+    pairs: hot and cold walls with buoyancy; the main lattice in FP32 or FP16
+    (a 16-bit lattice with walls above 384 MB pairs by default)."""
+    say(f"case: pairs thermal {vs} {storage}")
+    g, o = make_pair(bfa, orc, (12, 10, 10), vs, storage, 2, periodic=(1, 0, 1), thermal=True)
     assert g.info()["thermal"] == 1
     for n in (6, 3):
         g.step(n, save=True)
         o.step(n, save=True)
-        check(g, o, "fp32", vs, orc, thermal=True)
+        check(g, o, storage, vs, orc, thermal=True)
 
 
 def test_pairs_conservation_sums(bfa, orc):
