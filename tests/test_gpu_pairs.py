@@ -102,6 +102,22 @@ def test_pairs_walls_and_open_faces_match_oracle(bfa, orc, storage, periodic):
         check(g, o, storage, "D3Q19", orc)
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("periodic", [(1, 1, 0), (0, 1, 1), (1, 0, 1)])
+def test_pairs_open_faces_without_walls_match_oracle(bfa, orc, storage, periodic):
+    """No solid cells, one open axis (0.01 reads and dropped writes at its
+    faces; on z they cross chunk boundaries): the plain kernels with
+    neighbour checks, which a lattice of this kind above 384 MB pairs by
+    default."""
+    res = (13, 9, 11)
+    say(f"case: pairs open faces {storage} periodic={periodic}")
+    g, o = make_pair(bfa, orc, res, "D3Q19", storage, 3, periodic=periodic)
+    for n in (4, 3):
+        g.step(n, save=True)
+        o.step(n, save=True)
+        check(g, o, storage, "D3Q19", orc)
+
+
 @pytest.mark.parametrize("vs,storage", [("D3Q19", "fp32"), ("D3Q27", "fp32"), ("D3Q19", "fp16")])
 def test_pairs_thermal_match_oracle(bfa, orc, vs, storage):
     """The thermal lattice (D3Q7, FP32, in place like the main lattice) in
