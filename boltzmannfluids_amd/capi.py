@@ -21,6 +21,7 @@ PARAMS = {
     "thermal_expansion": 3, "thermal_relaxation_time": 4, "reference_temperature": 5,
     "thermal_cs2": 6, "velocity_limit": 7, "velocity_limit_extreme": 8,
     "shan_chen_synchronous": 9, "tick_pair_planes": 10,
+    "tick_sweep_ticks": 11,
 }
 FIELDS = {"velocity_density": 0, "boundary": 1, "force_temperature": 2,
           "populations_raw": 3, "thermal_raw": 4, "psi": 5, "populations": 6}

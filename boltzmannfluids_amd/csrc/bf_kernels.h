@@ -139,14 +139,23 @@ template <int VS>
 hipError_t launch_density(const StepArgs &a, int storage, bool odd, bool walls, const LaunchShape &shape,
                           hipStream_t stream);
 
-// One launch of a chunked tick pair (non-multiphase kernels): the even tick
-// on `nze` planes from ae.z0 and the odd tick on `nzo` planes from ao.z0, in
-// one grid.  The two plane ranges must not overlap, and the even range must
-// not reach within one plane of the odd range (see k_pair).  `partial`
-// receives the odd tick's FP64 block sums when non-null.
+// Plane ranges of one launch of a chunked sweep, in dispatch order: an even
+// tick, the odd tick after it, and for a sweep of two tick pairs the next even
+// and odd ticks.  Range k covers n[k] planes from z0[k], modulo Z.
+struct PairStages {
+    int z0[4];
+    int n[4];
+};
+
+// One launch of a chunked sweep (non-multiphase kernels): the ranges of `ps`
+// in one grid, the even ranges with `ae`, the odd ranges with `ao`.  A range
+// must not run beside another that touches its planes or their neighbours
+// (the host's schedule keeps every tick of a plane in a later launch than
+// the ticks it depends on; see k_pair).  `partial` receives the odd ticks'
+// FP64 block sums when non-null.
 template <int VS>
-hipError_t launch_pair(const StepArgs &ae, const StepArgs &ao, int nze, int nzo, int storage, bool walls, bool thermal,
-                       const LaunchShape &shape, double *partial, hipStream_t stream);
+hipError_t launch_pair(const StepArgs &ae, const StepArgs &ao, const PairStages &ps, int storage, bool walls,
+                       bool thermal, const LaunchShape &shape, double *partial, hipStream_t stream);
 
 // Neighbour mask of every cell: bit 0 = the cell is solid, bit i (1..Q-1) =
 // the neighbour at x + c_i lies inside the domain (periodic axes wrap) and is

@@ -63,6 +63,7 @@ struct bf_solver {
     float ulim = 0.25f, uext = 0.30f;
     int sc_sync = 0;  // synchronous Shan-Chen pseudopotential (density pre-pass per tick)
     int pair_planes = 0;  // chunked tick pairs: 0 automatic, -1 off, k > 0 chunks of k planes
+    int sweep_ticks = 0;  // ticks per chunked sweep: 0 automatic, 2 (one pair) or 4 (two pairs)
     float fconst[3] = {0, 0, 0};
     int forcing = 0, thermal = 0, has_ids = 0, initialized = 0;
     int closed = 0;  // see StepArgs::closed
@@ -692,6 +693,11 @@ bf_status bf_set_param(bf_solver *s, int32_t key, const float *v, int32_t n) {
             return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: tick-pair chunk must be -1, 0 or a plane count");
         s->pair_planes = (int)v[0];
         return BF_OK;
+    case BF_PARAM_TICK_SWEEP_TICKS:
+        if (!(v[0] == 0.0f || v[0] == 2.0f || v[0] == 4.0f))
+            return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: ticks per chunked sweep must be 0, 2 or 4");
+        s->sweep_ticks = (int)v[0];
+        return BF_OK;
     }
     return fail(s, BF_ERR_INVALID, "[LBM Error] bf_set_param: unknown key %d", key);
 }
@@ -712,6 +718,7 @@ bf_status bf_get_param(const bf_solver *s, int32_t key, float *v, int32_t n) {
     case BF_PARAM_VELOCITY_LIMIT_EXTREME: v[0] = s->uext; return BF_OK;
     case BF_PARAM_SHAN_CHEN_SYNCHRONOUS: v[0] = (float)s->sc_sync; return BF_OK;
     case BF_PARAM_TICK_PAIR_PLANES: v[0] = (float)s->pair_planes; return BF_OK;
+    case BF_PARAM_TICK_SWEEP_TICKS: v[0] = (float)s->sweep_ticks; return BF_OK;
     }
     return BF_ERR_INVALID;
 }
@@ -741,8 +748,27 @@ namespace {
 // (28 planes 675.1-675.3, whole-grid ticks 703.9-706.2).  The automatic mode
 // pairs the ticks of every lattice above kPairMinBytes, in chunks of about
 // kPairChunkBytes.
+//
+// Sweeps of two pairs (BF_PARAM_TICK_SWEEP_TICKS 4) hold four ticks' lines in
+// the cache at once and want smaller chunks (D3Q19, device time per tick,
+// interleaved on one solver, three rounds each): 256^3 FP32 370.1-370.2 us
+// with 16-plane chunks (80 MB; 12 planes 377, 20 planes 378, 24 planes 399)
+// against 371.7-372.0 for pairs; 256^3 FP16 194.0-194.2 us with 32 planes
+// (80 MB; 40-64 planes 195.3-196.5) against 195.8-199.2; the 384^3 porous
+// duct no faster (FP16 829.6-850.2 us with 14 planes against 834.1-842.2,
+// FP32 1472.9 with 5 planes against 1470.1-1472.8).  The automatic mode runs
+// two-pair sweeps in chunks of about kQuadChunkBytes for the 16-bit lattices
+// without walls or a thermal lattice, where the gain stands clear of the
+// noise; pairs otherwise.
 constexpr double kPairChunkBytes = 120e6;
+constexpr double kQuadChunkBytes = 80e6;
 constexpr double kPairMinBytes = 384e6;
+
+// ticks per chunked sweep (2 or 4)
+int sweep_depth(const bf_solver *s) {
+    if (s->sweep_ticks != 0) return s->sweep_ticks;
+    return (s->pair_planes == 0 && s->storage != ST_FP32 && !s->has_ids && !s->thermal) ? 4 : 2;
+}
 
 // planes per chunk, or 0 for whole-grid ticks
 int pair_chunk(const bf_solver *s) {
@@ -751,83 +777,128 @@ int pair_chunk(const bf_solver *s) {
     if (zc == 0) {
         const double plane = (double)s->X * s->Y * ((double)s->q * s->elem + (s->thermal ? 4.0 * s->qt : 0.0));
         if (plane * s->Z < kPairMinBytes) return 0;
-        zc = std::max(1, (int)std::lround(kPairChunkBytes / plane));
+        const double chunk = sweep_depth(s) == 4 ? kQuadChunkBytes : kPairChunkBytes;
+        zc = std::max(1, (int)std::lround(chunk / plane));
     }
     // the schedule needs at least four chunks (chunk 0's odd tick waits for
     // the last chunk's even tick, which writes into plane 0 on a periodic z axis)
     return (s->Z + zc - 1) / zc >= 4 ? zc : 0;
 }
 
-hipError_t dispatch_pair(const bf_solver *s, const StepArgs &ae, const StepArgs &ao, int nze, int nzo,
+hipError_t dispatch_pair(const bf_solver *s, const StepArgs &ae, const StepArgs &ao, const PairStages &ps,
                          double *partial) {
     hipStream_t stream = s->stream;
     const bool walls = s->has_ids != 0, th = s->thermal != 0;
     switch (s->vs) {
-    case D2Q9: return launch_pair<D2Q9>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
-    case D3Q15: return launch_pair<D3Q15>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
-    case D3Q19: return launch_pair<D3Q19>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
-    case D3Q27: return launch_pair<D3Q27>(ae, ao, nze, nzo, s->storage, walls, th, s->shape, partial, stream);
+    case D2Q9: return launch_pair<D2Q9>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
+    case D3Q15: return launch_pair<D3Q15>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
+    case D3Q19: return launch_pair<D3Q19>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
+    case D3Q27: return launch_pair<D3Q27>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
     }
     return hipErrorInvalidValue;
 }
 
-// An even tick and the odd tick after it, in chunks of zc planes (m >= 4
-// chunks), on the handle's stream: m + 1 launches
+// An even tick and the odd tick after it (depth 2), or two such pairs
+// (depth 4), in chunks of zc planes (m >= 4 chunks), on the handle's stream.
+// Depth 2, m + 1 launches:
 //   E0 | E1 + O[1, lo(1) - 1) | ... | E(m-1) + O[.., lo(m-1) - 1) | O[lo(m-1) - 1, Z) O0
 // Within a launch the odd planes end two planes before the even chunk: even
 // ticks of different cells, or odd ticks, never touch one another's slots.
 // The odd planes read what the even chunk of the previous launch wrote, which
-// the last-level cache still holds.  `flags` apply to the odd tick.
-bf_status run_pair(bf_solver *s, int zc, uint32_t flags) {
+// the last-level cache still holds.  Depth 4 adds the second pair's even tick
+// E' and odd tick O' to the same launches (m + 3 launches), each two planes
+// behind the tick before it, so each tick of a plane again reads what the
+// previous launch wrote.  `flags` apply to the last odd tick; depth 4 is run
+// without flags (run_ticks ends a call that saves or sums with a pair).
+bf_status run_pair(bf_solver *s, int zc, int depth, uint32_t flags) {
     const int Z = s->Z, m = (Z + zc - 1) / zc;
     StepArgs ae = make_args(s, true), ao = make_args(s, true);
     if (flags & BF_STEP_SAVE_MACRO) { ao.vd = s->vd; ao.ft = s->ft; }
     double *partial = (flags & BF_STEP_CONSERVATION) ? s->partial : nullptr;
     auto lo = [&](int j) { return j * zc; };
     auto hi = [&](int j) { return std::min(Z, (j + 1) * zc); };
-    // even tick on planes [ze0, ze0 + nze), odd tick on planes zo0, zo0 + 1, ...
-    // (nzo of them, modulo Z)
-    auto launch = [&](int ze0, int nze, int zo0, int nzo) -> bf_status {
-        ae.z0 = ze0;
-        ao.z0 = zo0;
-        const hipError_t e = dispatch_pair(s, ae, ao, nze, nzo, partial);
+    // ranges in order E, O, E', O': first plane and plane count (modulo Z)
+    auto launch = [&](int e0, int ne, int o0, int no, int e1, int ne1, int o1, int no1) -> bf_status {
+        const PairStages ps{{e0, o0, e1, o1}, {ne, no, ne1, no1}};
+        const hipError_t e = dispatch_pair(s, ae, ao, ps, partial);
         if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] tick-pair launch failed: %s", hipGetErrorString(e));
         return BF_OK;
     };
-    // The odd tick one plane behind the even tick: launch j runs the even
-    // tick on chunk j and the odd tick on every plane p in 1 .. lo(j) - 2 not
-    // yet done (plane p needs the even ticks of p - 1, p and p + 1, all in
-    // earlier launches; this launch's even chunk reads and writes from plane
-    // lo(j) - 1 on).  Plane 0 and the planes from lo(m - 1) - 1 come last: on
-    // a periodic z axis the last chunk's even tick writes into plane 0.
-    bf_status st = launch(0, hi(0), 0, 0);
-    int zo = 1;
-    for (int j = 1; j < m && st == BF_OK; ++j) {
-        const int nzo = std::max(0, lo(j) - 1 - zo);
-        st = launch(lo(j), hi(j) - lo(j), zo, nzo);
-        zo += nzo;
+    bf_status st = BF_OK;
+    if (depth == 2) {
+        // The odd tick one plane behind the even tick: launch j runs the even
+        // tick on chunk j and the odd tick on every plane p in 1 .. lo(j) - 2
+        // not yet done (plane p needs the even ticks of p - 1, p and p + 1, all
+        // in earlier launches; this launch's even chunk reads and writes from
+        // plane lo(j) - 1 on).  Plane 0 and the planes from lo(m - 1) - 1 come
+        // last: on a periodic z axis the last chunk's even tick writes into
+        // plane 0.
+        st = launch(0, hi(0), 0, 0, 0, 0, 0, 0);
+        int zo = 1;
+        for (int j = 1; j < m && st == BF_OK; ++j) {
+            const int nzo = std::max(0, lo(j) - 1 - zo);
+            st = launch(lo(j), hi(j) - lo(j), zo, nzo, 0, 0, 0, 0);
+            zo += nzo;
+        }
+        if (st == BF_OK) st = launch(0, 0, zo, Z - zo + 1, 0, 0, 0, 0);
+    } else {
+        // Each tick after the first covers, per launch, the planes that the
+        // tick before it finished in earlier launches, up to two planes short
+        // of that tick's first plane in this launch (a plane's tick needs the
+        // tick before it on the plane and both neighbours; an even tick reads
+        // and writes the neighbour planes).  O starts at plane 1, E' at 2, O'
+        // at 3: plane 0 waits for the last chunk's even tick, which writes into
+        // it on a periodic z axis.  The last three launches finish O, E' and O'
+        // over the planes from their fronts to Z - 1 and then 0, 0 .. 1 and
+        // 0 .. 2.
+        int zo = 1, ze1 = 2, zo1 = 3;
+        for (int j = 0; j < m && st == BF_OK; ++j) {
+            const int no = j == 0 ? 0 : std::max(0, lo(j) - 1 - zo);
+            const int ne1 = std::max(0, zo - 1 - ze1);
+            const int no1 = std::max(0, ze1 - 1 - zo1);
+            st = launch(lo(j), hi(j) - lo(j), zo, no, ze1, ne1, zo1, no1);
+            zo += no; ze1 += ne1; zo1 += no1;
+        }
+        if (st == BF_OK) {
+            const int ne1 = std::max(0, zo - 1 - ze1), no1 = std::max(0, ze1 - 1 - zo1);
+            st = launch(0, 0, zo, Z - zo + 1, ze1, ne1, zo1, no1);
+            ze1 += ne1; zo1 += no1;
+        }
+        if (st == BF_OK) {
+            const int no1 = std::max(0, ze1 - 1 - zo1);
+            st = launch(0, 0, 0, 0, ze1, Z - ze1 + 2, zo1, no1);
+            zo1 += no1;
+        }
+        if (st == BF_OK) st = launch(0, 0, 0, 0, 0, 0, zo1, Z - zo1 + 3);
     }
-    if (st == BF_OK) st = launch(0, 0, zo, Z - zo + 1);
     if (st != BF_OK) return st;
     if (partial) {
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s->stream, s->partial, s->partial_blocks, s->sums_d);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(s, BF_ERR_DEVICE, "[LBM Error] reduction launch failed: %s", hipGetErrorString(e));
     }
-    s->ticks += 2;
+    s->ticks += depth;
     return BF_OK;
 }
 
 }  // namespace
 
 static bf_status run_ticks(bf_solver *s, int64_t n, uint32_t flags) {
-    const int zc = pair_chunk(s);
+    const int zc = pair_chunk(s), depth = sweep_depth(s);
     for (int64_t k = 0; k < n; ++k) {
         const bool last = k == n - 1;
         const bool odd = ((s->ticks - s->tick_base) & 1) != 0;
+        if (zc > 0 && !odd && depth == 4 && (k + 4 < n || (k + 4 == n && flags == 0u))) {
+            // two pairs of ticks, chunk by chunk (a call that saves or sums
+            // ends with a single pair, which takes the flags)
+            const bf_status st = run_pair(s, zc, 4, 0u);
+            if (st != BF_OK) return st;
+            k += 3;
+            continue;
+        }
         if (zc > 0 && !odd && k + 1 < n) {
             // an even tick and the odd tick after it, chunk by chunk
-            const bf_status st = run_pair(s, zc, k + 2 == n ? flags : 0u);
+            const bf_status st = run_pair(s, zc, 2, k + 2 == n ? flags : 0u);
             if (st != BF_OK) return st;
             ++k;
             continue;

@@ -943,12 +943,13 @@ __global__ void __launch_bounds__(512) k_density(const StepArgs a) {
         a, nullptr, grid_block(a, 1, a.z0 + (int)blockIdx.z));
 }
 
-// A launch of a chunked tick pair: the blocks of the first `nze` planes run
-// the even tick on planes [ae.z0, ae.z0 + nze), the others the odd tick on
-// planes from ao.z0.  The host schedules chunk j's even tick with the odd
-// tick of the planes that end two planes before the chunk, so the two halves
-// touch disjoint planes (the even tick of a plane reads and writes its
-// neighbour planes z +- 1; the odd tick stays at its cells).  The even tick's
+// A launch of a chunked sweep: the blocks run the plane ranges of `ps` in
+// order, even tick, odd tick, and for a sweep of two pairs the next even and
+// odd ticks.  The host schedules chunk j's even tick with the odd tick of the
+// planes that end two planes before the chunk (and the next pair's ticks two
+// planes behind each other), so the ranges touch disjoint planes (the even
+// tick of a plane reads and writes its neighbour planes z +- 1; the odd tick
+// stays at its cells).  The even tick's
 // accesses are ordinary loads and stores, whose lines the 256 MB last-level
 // cache keeps; the odd tick of those planes, one launch later, finds most of
 // them there and streams its own traffic with nontemporal hints.  (Which hints the two halves use measured within noise
@@ -968,20 +969,25 @@ __global__ void __launch_bounds__(512) k_density(const StepArgs a) {
 #endif
 #define BF_OCCUPANCY_PAIR __attribute__((amdgpu_waves_per_eu(TH ? 1 : BF_WAVES_PER_EU)))
 template <int VS, int ST, bool WALLS, bool TH, int CPT, bool AL, bool PER, int SRC>
-__global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs ae, const StepArgs ao, const int nze,
+__global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs ae, const StepArgs ao, const PairStages ps,
                                                                 double *__restrict__ partial) {
     // the even chunk's planes come first in dispatch order (alternating the
     // planes of the two chunks measured 2-8 % slower)
-    const bool even = (int)blockIdx.z < nze;
-    const int zl = even ? (int)blockIdx.z : (int)blockIdx.z - nze;
+    const int bz = (int)blockIdx.z;
+    const int b1 = ps.n[0], b2 = b1 + ps.n[1], b3 = b2 + ps.n[2];
+    const bool even = bz < b1 || (bz >= b2 && bz < b3);
+    const int z = bz < b1 ? ps.z0[0] + bz
+                : bz < b2 ? ps.z0[1] + (bz - b1)
+                : bz < b3 ? ps.z0[2] + (bz - b2)
+                          : ps.z0[3] + (bz - b3);
+    // the last launches of a sweep wrap from plane Z - 1 to 0
+    const int zw = z < ae.Z ? z : z - ae.Z;
     if (even) {
         step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, SRC, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
-            ae, nullptr, grid_block(ae, CPT, ae.z0 + zl));
+            ae, nullptr, grid_block(ae, CPT, zw));
     } else {
-        // the odd planes may wrap from Z - 1 to 0 (the last launch of a pair)
-        const int z = ao.z0 + zl;
         step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, SRC, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
-            ao, partial, grid_block(ao, CPT, z < ao.Z ? z : z - ao.Z));
+            ao, partial, grid_block(ao, CPT, zw));
     }
 }
 
@@ -1116,7 +1122,7 @@ hipError_t launch_step_st(const StepArgs &a_in, bool walls, bool mp, bool th, co
 }
 
 template <int ST>
-hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, int nze, int nzo, bool walls, bool th,
+hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, const PairStages &ps, bool walls, bool th,
                           const LaunchShape &sh, double *partial, hipStream_t stream) {
     constexpr int VS = BF_VS;
     StepArgs ae = ae_in, ao = ao_in;
@@ -1125,12 +1131,12 @@ hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, int nze,
     constexpr int CPT = cells_per_thread(ST, false, false);
     // as for single ticks: the row buffer for the plain kernels and FP32 walls
     const bool al = sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
-    const dim3 grid(sh.grid.x, sh.grid.y, (unsigned)(nze + nzo));
+    const dim3 grid(sh.grid.x, sh.grid.y, (unsigned)(ps.n[0] + ps.n[1] + ps.n[2] + ps.n[3]));
     const size_t lds = al ? sh.lds_bytes : 0;
 #define BF_P2(W, T, A, P, S)                                                                               \
     do {                                                                                                   \
         hipLaunchKernelGGL((k_pair<VS, ST, W, T, T ? 1 : CPT, A, P, S>), grid, sh.block, lds, stream, ae, ao, \
-                           nze, partial);                                                                  \
+                           ps, partial);                                                                   \
         return hipGetLastError();                                                                          \
     } while (0)
 #define BF_P(W, T, A, P)                                                                                   \
@@ -1215,12 +1221,12 @@ hipError_t launch_density<BF_VS>(const StepArgs &a, int storage, bool odd, bool 
 }
 
 template <>
-hipError_t launch_pair<BF_VS>(const StepArgs &ae, const StepArgs &ao, int nze, int nzo, int storage, bool walls,
+hipError_t launch_pair<BF_VS>(const StepArgs &ae, const StepArgs &ao, const PairStages &ps, int storage, bool walls,
                               bool th, const LaunchShape &sh, double *partial, hipStream_t stream) {
     switch (storage) {
-    case ST_FP32: return launch_pair_st<ST_FP32>(ae, ao, nze, nzo, walls, th, sh, partial, stream);
-    case ST_FP16: return launch_pair_st<ST_FP16>(ae, ao, nze, nzo, walls, th, sh, partial, stream);
-    case ST_BF16: return launch_pair_st<ST_BF16>(ae, ao, nze, nzo, walls, th, sh, partial, stream);
+    case ST_FP32: return launch_pair_st<ST_FP32>(ae, ao, ps, walls, th, sh, partial, stream);
+    case ST_FP16: return launch_pair_st<ST_FP16>(ae, ao, ps, walls, th, sh, partial, stream);
+    case ST_BF16: return launch_pair_st<ST_BF16>(ae, ao, ps, walls, th, sh, partial, stream);
     }
     return hipErrorInvalidValue;
 }

@@ -118,7 +118,18 @@ typedef enum bf_param {
      * identical in every mode (each cell sees the same operations); the
      * Shan-Chen solver always runs whole-grid ticks, and so does a grid of
      * fewer than four chunks.  No counterpart in the reference. */
-    BF_PARAM_TICK_PAIR_PLANES = 10
+    BF_PARAM_TICK_PAIR_PLANES = 10,
+    /* 1 float, 0, 2 or 4 (default 0): ticks per chunked sweep.  4 runs two
+     * tick pairs in one sweep over the chunks: each launch also carries the
+     * next pair's even tick and odd tick on the planes the tick before each
+     * has finished, so every tick of a plane reads what the previous launch
+     * wrote.  A bf_step call that saves fields or sums on its last tick ends
+     * with a single pair.  0 = 4 with chunks of about 80 MB for 16-bit
+     * lattices without boundary ids or a thermal lattice when
+     * BF_PARAM_TICK_PAIR_PLANES is automatic (where it was measured faster),
+     * else 2.  Results are identical in every mode.  No counterpart in the
+     * reference. */
+    BF_PARAM_TICK_SWEEP_TICKS = 11
 } bf_param;
 
 /* Fields for bf_read_field. */

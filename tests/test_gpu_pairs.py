@@ -2,6 +2,7 @@
 after it run z-chunk by z-chunk, chunk j's even tick in one launch with chunk
 j-2's odd tick.  Every cell sees the same operations as with whole-grid ticks,
 so the results must be bit-identical to the whole-grid ticks and to the oracle
+(sweeps of two pairs, BF_PARAM_TICK_SWEEP_TICKS, likewise)
 (FP32), or identical to the oracle's emulated 16-bit storage up to one-quantum
 rounding flips, whatever the chunk size: chunks of 1, 2 and 3 planes, a ragged
 last chunk, both parities at the start and end of a bf_step call, saves and
@@ -21,7 +22,8 @@ def say(*a):
     sys.stdout.flush()
 
 
-def make_pair(bfa, orc, res, vs, storage, planes, *, periodic=(1, 1, 1), walls=False, thermal=False, seed=SEED + 51):
+def make_pair(bfa, orc, res, vs, storage, planes, *, periodic=(1, 1, 1), walls=False, thermal=False, seed=SEED + 51,
+              sweep=0):
     X, Y, Z = res
     n = X * Y * Z
     rho, u = random_state(n, seed=seed, du=0.06, drho=0.04)
@@ -43,6 +45,7 @@ def make_pair(bfa, orc, res, vs, storage, planes, *, periodic=(1, 1, 1), walls=F
     g = bfa.Solver(res, 0.6, periodic=periodic, velocity_set=vs, storage=storage)
     o = orc.Oracle(res, 0.6, periodic=periodic, velocity_set=vs, storage=storage, precision="f32")
     g.set_param("tick_pair_planes", planes)
+    g.set_param("tick_sweep_ticks", sweep)
     for oid, kw in objects:
         g.set_object(oid, **kw)
         o.set_object(oid, **kw)
@@ -132,6 +135,35 @@ def test_pairs_thermal_match_oracle(bfa, orc, vs, storage):
         check(g, o, storage, vs, orc, thermal=True)
 
 
+SWEEP_CASES = [("D3Q19", "fp32", 1, (12, 10, 8), {}), ("D3Q19", "fp32", 3, (12, 10, 13), {}),
+               ("D3Q19", "fp16", 2, (16, 6, 9), {}), ("D3Q19", "bf16", 2, (16, 6, 9), {}),
+               ("D3Q15", "fp32", 2, (10, 8, 8), {}), ("D3Q27", "fp16", 3, (10, 8, 12), {}),
+               ("D3Q19", "fp32", 2, (64, 5, 9), {}),
+               ("D3Q19", "fp32", 2, (14, 9, 11), dict(periodic=(1, 0, 0), walls=True)),
+               ("D3Q19", "fp16", 2, (14, 9, 11), dict(periodic=(0, 1, 0), walls=True)),
+               ("D3Q19", "bf16", 3, (13, 9, 11), dict(periodic=(1, 1, 0))),
+               ("D3Q19", "fp32", 2, (12, 10, 10), dict(periodic=(1, 0, 1), thermal=True)),
+               ("D3Q27", "fp16", 2, (12, 10, 10), dict(periodic=(1, 0, 1), thermal=True))]
+
+
+@pytest.mark.parametrize("vs,storage,planes,res,kw", SWEEP_CASES)
+def test_two_pair_sweeps_match_oracle(bfa, orc, vs, storage, planes, res, kw):
+    """Sweeps of two tick pairs (BF_PARAM_TICK_SWEEP_TICKS 4): each launch
+    also runs the next pair's even and odd ticks two planes behind the tick
+    before each.  Calls of 5, 8, 3 and 6 ticks (sweeps start on both
+    parities; calls that save end with a pair or a single tick, one without
+    saves ends with a sweep), against the oracle."""
+    say(f"case: two-pair sweeps {vs} {storage} {res} planes={planes} {kw}")
+    g, o = make_pair(bfa, orc, res, vs, storage, planes, sweep=4, **kw)
+    for n, save in ((5, True), (8, False), (3, True), (6, True)):
+        g.step(n, save=save)
+        o.step(n, save=save)
+    if storage == "fp32":
+        g.step(1, save=True)
+        o.step(1, save=True)
+    check(g, o, storage, vs, orc, thermal=kw.get("thermal", False))
+
+
 def test_pairs_conservation_sums(bfa, orc):
     """FP64 sums recorded on the last tick of a pair: each odd chunk writes
     the partials of its own planes."""
@@ -163,8 +195,9 @@ def test_pairs_equal_whole_grid_ticks_porous_96(bfa, storage):
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 def test_automatic_mode_on_the_periodic_tgv_256(bfa, storage):
     """The automatic mode pairs the ticks of the 256^3 box (1.3 GB in FP32,
-    0.64 GB in FP16; no walls), bit-identical to whole-grid ticks (6 ticks,
-    saved fields on the last)."""
+    0.64 GB in FP16; no walls; FP16 in sweeps of two pairs), bit-identical to
+    whole-grid ticks (6 ticks, saved fields on the last: FP16 runs one sweep
+    of two pairs, then a pair)."""
     from boltzmannfluids_amd.scenarios import taylor_green
     n = 256
     say(f"case: automatic tick pairs TGV {n}^3 {storage}")
@@ -188,3 +221,9 @@ def test_pairs_parameter(bfa):
     for bad in (-2, 1.5):
         with pytest.raises(bfa.BFError):
             s.set_param("tick_pair_planes", bad)
+    assert s.get_param("tick_sweep_ticks") == 0
+    s.set_param("tick_sweep_ticks", 4)
+    assert s.get_param("tick_sweep_ticks") == 4
+    for bad in (1, 3, -1, 8):
+        with pytest.raises(bfa.BFError):
+            s.set_param("tick_sweep_ticks", bad)
