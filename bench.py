@@ -288,10 +288,15 @@ def adjoint_ratio(bfa, res, vs, steps=2000, k=100, budget=0):
         n *= r
     state_bytes = 4 * (19 if vs == "D3Q19" else 9) * ((n + 63) // 64 * 64)
     resident = min(steps // k, max(1, budget // (state_bytes * k))) if budget else 0
+    # the forward pass keeps its states for the resident segments (at least
+    # one) and, when others are recomputed, for the segment before them
+    nseg = (steps + k - 1) // k
+    kept = max(1, resident)
+    kept = nseg if kept >= nseg else kept + 1
     return {"workload": f"channel {vs} {'x'.join(map(str, res))}, tau 0.6, u_in 0.05, porosity 0.02 x box-smoothed "
                         f"U[0,1], {steps} ticks, checkpoint every {k}",
             "trajectory_budget_GB": round(budget / 1e9, 1), "resident_segments": resident,
-            "segments": steps // k,
+            "segments": steps // k, "segments_recomputed": nseg - kept,
             "J": J, "forward_ms": round(fwd, 3), "gradient_ms": round(t[2], 3),
             "recompute_ms": round(t[1], 3), "recompute_beside_sweep": True,
             "ratio": round(t[2] / fwd, 3), "dJ_duin": du}, dg

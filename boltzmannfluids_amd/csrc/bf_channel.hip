@@ -674,9 +674,12 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
     resident = std::min(resident, nseg);
     // Trajectory slots: one per resident segment, plus one when segments are
     // recomputed, so that the recomputation of the next segment (second
-    // stream) runs beside the adjoint sweep of the current one.
+    // stream) runs beside the adjoint sweep of the current one.  The forward
+    // pass fills that extra slot too, with the last segment before the
+    // resident ones, which is then not recomputed.
     const int64_t nrec = nseg - resident;
     const int64_t nslots = resident + (nrec > 0 ? 1 : 0);
+    const int64_t first_kept = nseg - nslots;   // segments from here keep the forward pass's states
     Scratch sc;
     float *ckpt = nullptr, *traj = nullptr, *nu[2] = {nullptr, nullptr}, *dgseg = nullptr;
     double *dg64 = nullptr, *du_rows = nullptr;
@@ -691,12 +694,15 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
     const int64_t qs = (int64_t)c->q * c->stride;
     // Segment k covers ticks [kK, min(T, (k+1)K)).  State j of segment k is
     // s^{kK+j-1}, the state tick kK+j reads: j = 0 is checkpoint k, j >= 1 live
-    // in the segment's trajectory slot (resident segments keep their own; a
-    // recomputed segment takes the extra slot or slot 0, whichever holds no
-    // segment still to be swept).
-    auto resident_slot = [&](int64_t k) { return k >= nseg - resident ? k - (nseg - resident) : -1; };
+    // in the segment's trajectory slot (resident segments keep their own, the
+    // segment before them the extra slot; a recomputed segment takes the
+    // extra slot or slot 0, whichever holds no segment still to be swept).
+    auto resident_slot = [&](int64_t k) {
+        if (k >= nseg - resident) return k - (nseg - resident);
+        return k == first_kept ? resident : (int64_t)-1;   // the extra slot
+    };
     std::vector<int64_t> seg_slot((size_t)nseg, -1);
-    for (int64_t k = nseg - resident; k < nseg; ++k) seg_slot[(size_t)k] = resident_slot(k);
+    for (int64_t k = first_kept; k < nseg; ++k) seg_slot[(size_t)k] = resident_slot(k);
     auto seg_state = [&](int64_t k, int64_t j) -> float * {
         if (j == 0) return ckpt + k * qs;
         return traj + (seg_slot[(size_t)k] * per_seg + (j - 1)) * qs;
@@ -747,10 +753,10 @@ bf_status bf_channel_gradient(bf_channel *c, const bf_adjoint_config *acfg, doub
     hipEvent_t fwd_done;
     if (sc.event(&fwd_done) != hipSuccess) return cfail(c, BF_ERR_DEVICE, "[LBM Error] bf_channel_gradient: cannot create events");
     std::vector<int64_t> owner((size_t)nslots, -1);   // segment held by the slot and not swept yet
-    for (int64_t k = nseg - resident; k < nseg; ++k) owner[(size_t)resident_slot(k)] = k;
+    for (int64_t k = first_kept; k < nseg; ++k) owner[(size_t)resident_slot(k)] = k;
     std::vector<char> released((size_t)nslots, 0);     // slot_free recorded at least once
     std::vector<int64_t> rec_of((size_t)nseg, -1);     // index of the segment's event pair
-    int64_t k_rec = nseg - resident - 1, n_rec = 0;     // next segment to recompute (descending)
+    int64_t k_rec = first_kept - 1, n_rec = 0;         // next segment to recompute (descending)
     CH_HIP(c, hipEventRecord(fwd_done, c->stream));
     CH_HIP(c, hipStreamWaitEvent(c->stream2, fwd_done, 0));
     auto try_recompute = [&]() -> hipError_t {

@@ -66,10 +66,12 @@ def test_channel_gradient_vs_fp64_adjoint(bfa, orc, vs, res, T, K):
 
 
 def test_checkpointing_is_exact(bfa, orc):
-    """All segments but the last recomputed, two or three resident, all
-    resident: identical results.  The recomputations run on a second stream
-    beside the adjoint sweep, alternating between an extra trajectory slot and
-    the slot of the last resident segment once that segment is swept."""
+    """All segments but the last two recomputed (the forward pass fills the
+    resident slot and the extra slot), two or three resident, all resident:
+    identical results.  The recomputations run on a second stream beside the
+    adjoint sweep, alternating between the extra trajectory slot and the slot
+    of the last resident segment once that segment is swept.  With two
+    segments nothing is recomputed."""
     res, T, K = (80, 20, 1), 230, 40
     g, _, _ = make_pair(bfa, orc, res, "D2Q9")
     seg = 9 * ((80 * 20 + 63) // 64 * 64) * 4 * (K - 1)
@@ -78,6 +80,9 @@ def test_checkpointing_is_exact(bfa, orc):
         assert J == out[0][0]
         np.testing.assert_array_equal(dg, out[0][1])
         assert du == out[0][2]
+    two = [g.gradient(70, checkpoint_every=K, trajectory_budget_bytes=b) for b in (0, 100 * seg)]
+    assert two[0][0] == two[1][0] and two[0][2] == two[1][2]
+    np.testing.assert_array_equal(two[0][1], two[1][1])
 
 
 def test_full_size_gradient_vs_central_differences(bfa, orc):
