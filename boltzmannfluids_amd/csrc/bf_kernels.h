@@ -147,7 +147,8 @@ struct PairStages {
     int n[4];
 };
 
-// One launch of a chunked sweep (non-multiphase kernels): the ranges of `ps`
+// One launch of a chunked sweep (multiphase: the lagged Shan-Chen kernels,
+// whose psi buffers `ae` and `ao` name in tick order): the ranges of `ps`
 // in one grid, the even ranges with `ae`, the odd ranges with `ao`.  A range
 // must not run beside another that touches its planes or their neighbours
 // (the host's schedule keeps every tick of a plane in a later launch than
@@ -155,7 +156,7 @@ struct PairStages {
 // FP64 block sums when non-null.
 template <int VS>
 hipError_t launch_pair(const StepArgs &ae, const StepArgs &ao, const PairStages &ps, int storage, bool walls,
-                       bool thermal, const LaunchShape &shape, double *partial, hipStream_t stream);
+                       bool multiphase, bool thermal, const LaunchShape &shape, double *partial, hipStream_t stream);
 
 // Neighbour mask of every cell: bit 0 = the cell is solid, bit i (1..Q-1) =
 // the neighbour at x + c_i lies inside the domain (periodic axes wrap) and is

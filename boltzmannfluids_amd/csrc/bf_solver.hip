@@ -767,12 +767,19 @@ constexpr double kPairMinBytes = 384e6;
 // ticks per chunked sweep (2 or 4)
 int sweep_depth(const bf_solver *s) {
     if (s->sweep_ticks != 0) return s->sweep_ticks;
-    return (s->pair_planes == 0 && s->storage != ST_FP32 && !s->has_ids && !s->thermal) ? 4 : 2;
+    return (s->pair_planes == 0 && s->storage != ST_FP32 && !s->has_ids && !s->thermal && !s->cfg.multiphase) ? 4 : 2;
 }
 
 // planes per chunk, or 0 for whole-grid ticks
 int pair_chunk(const bf_solver *s) {
-    if (s->cfg.multiphase || s->pair_planes < 0) return 0;
+    // The synchronous Shan-Chen mode runs a density pre-pass before each
+    // tick.  The lagged mode pairs only on request: its even tick is bound by
+    // instruction issue, and pairs measured slower (D3Q27 + D3Q7 thermal
+    // walls, 192^3 FP32, device time per tick, three rounds: whole-grid ticks
+    // 369.8-371.1 us; pairs of 12, 16, 24 and 32 planes 425.6-426.1, 409.3-409.5,
+    // 391.0-391.1 and 384.9-385.0; two-pair sweeps of 8, 12 and 16 planes
+    // 401.7-401.9, 391.9-392.2 and 381.9-382.0).
+    if ((s->cfg.multiphase && (s->sc_sync || s->pair_planes == 0)) || s->pair_planes < 0) return 0;
     int zc = s->pair_planes;
     if (zc == 0) {
         const double plane = (double)s->X * s->Y * ((double)s->q * s->elem + (s->thermal ? 4.0 * s->qt : 0.0));
@@ -788,12 +795,12 @@ int pair_chunk(const bf_solver *s) {
 hipError_t dispatch_pair(const bf_solver *s, const StepArgs &ae, const StepArgs &ao, const PairStages &ps,
                          double *partial) {
     hipStream_t stream = s->stream;
-    const bool walls = s->has_ids != 0, th = s->thermal != 0;
+    const bool walls = s->has_ids != 0, mp = s->cfg.multiphase != 0, th = s->thermal != 0;
     switch (s->vs) {
-    case D2Q9: return launch_pair<D2Q9>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
-    case D3Q15: return launch_pair<D3Q15>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
-    case D3Q19: return launch_pair<D3Q19>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
-    case D3Q27: return launch_pair<D3Q27>(ae, ao, ps, s->storage, walls, th, s->shape, partial, stream);
+    case D2Q9: return launch_pair<D2Q9>(ae, ao, ps, s->storage, walls, mp, th, s->shape, partial, stream);
+    case D3Q15: return launch_pair<D3Q15>(ae, ao, ps, s->storage, walls, mp, th, s->shape, partial, stream);
+    case D3Q19: return launch_pair<D3Q19>(ae, ao, ps, s->storage, walls, mp, th, s->shape, partial, stream);
+    case D3Q27: return launch_pair<D3Q27>(ae, ao, ps, s->storage, walls, mp, th, s->shape, partial, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -813,6 +820,12 @@ hipError_t dispatch_pair(const bf_solver *s, const StepArgs &ae, const StepArgs 
 bf_status run_pair(bf_solver *s, int zc, int depth, uint32_t flags) {
     const int Z = s->Z, m = (Z + zc - 1) / zc;
     StepArgs ae = make_args(s, true), ao = make_args(s, true);
+    if (s->cfg.multiphase) {
+        // lagged psi: the odd tick reads what the even tick writes, and writes
+        // the buffer the even tick read (the next even tick reads it)
+        ao.psi_prev = s->psi[s->cur];
+        ao.psi_cur = s->psi[s->cur ^ 1];
+    }
     if (flags & BF_STEP_SAVE_MACRO) { ao.vd = s->vd; ao.ft = s->ft; }
     double *partial = (flags & BF_STEP_CONSERVATION) ? s->partial : nullptr;
     auto lo = [&](int j) { return j * zc; };

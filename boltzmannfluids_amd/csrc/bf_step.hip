@@ -967,8 +967,12 @@ __global__ void __launch_bounds__(512) k_density(const StepArgs a) {
 #ifndef BF_PAIR_ODD_NT_STORE
 #define BF_PAIR_ODD_NT_STORE 1
 #endif
-#define BF_OCCUPANCY_PAIR __attribute__((amdgpu_waves_per_eu(TH ? 1 : BF_WAVES_PER_EU)))
-template <int VS, int ST, bool WALLS, bool TH, int CPT, bool AL, bool PER, int SRC>
+#define BF_OCCUPANCY_PAIR __attribute__((amdgpu_waves_per_eu((MP || TH) ? 1 : BF_WAVES_PER_EU)))
+// MP: the lagged Shan-Chen kernels.  Tick t reads its neighbours' psi from
+// the buffer tick t - 1 wrote (radius one, like the populations) and writes
+// its own psi to the other buffer, so the planes that the schedule keeps
+// apart for the populations keep the psi reads and writes apart as well.
+template <int VS, int ST, bool WALLS, bool MP, bool TH, int CPT, bool AL, bool PER, int SRC>
 __global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs ae, const StepArgs ao, const PairStages ps,
                                                                 double *__restrict__ partial) {
     // the even chunk's planes come first in dispatch order (alternating the
@@ -983,10 +987,10 @@ __global__ void __launch_bounds__(512) BF_OCCUPANCY_PAIR k_pair(const StepArgs a
     // the last launches of a sweep wrap from plane Z - 1 to 0
     const int zw = z < ae.Z ? z : z - ae.Z;
     if (even) {
-        step_body<VS, ST, false, WALLS, false, TH, CPT, AL, PER, SRC, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
+        step_body<VS, ST, false, WALLS, MP, TH, CPT, AL, PER, SRC, BF_PAIR_EVEN_NT_LOAD, BF_PAIR_EVEN_NT_STORE>(
             ae, nullptr, grid_block(ae, CPT, zw));
     } else {
-        step_body<VS, ST, true, WALLS, false, TH, CPT, false, PER, SRC, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
+        step_body<VS, ST, true, WALLS, MP, TH, CPT, false, PER, SRC, BF_PAIR_ODD_NT_LOAD, BF_PAIR_ODD_NT_STORE>(
             ao, partial, grid_block(ao, CPT, zw));
     }
 }
@@ -1122,20 +1126,27 @@ hipError_t launch_step_st(const StepArgs &a_in, bool walls, bool mp, bool th, co
 }
 
 template <int ST>
-hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, const PairStages &ps, bool walls, bool th,
-                          const LaunchShape &sh, double *partial, hipStream_t stream) {
+hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, const PairStages &ps, bool walls, bool mp,
+                          bool th, const LaunchShape &sh, double *partial, hipStream_t stream) {
     constexpr int VS = BF_VS;
     StepArgs ae = ae_in, ao = ao_in;
     ae.bdx = ao.bdx = (int)sh.block.x;
     ae.bdy = ao.bdy = (int)sh.block.y;
     constexpr int CPT = cells_per_thread(ST, false, false);
     // as for single ticks: the row buffer for the plain kernels and FP32 walls
-    const bool al = sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
+    const bool al = !mp && sh.lds_bytes > 0 && !(walls && ST != ST_FP32 && !BF_ROWBUF_16BIT_WALLS);
     const dim3 grid(sh.grid.x, sh.grid.y, (unsigned)(ps.n[0] + ps.n[1] + ps.n[2] + ps.n[3]));
     const size_t lds = al ? sh.lds_bytes : 0;
 #define BF_P2(W, T, A, P, S)                                                                               \
     do {                                                                                                   \
-        hipLaunchKernelGGL((k_pair<VS, ST, W, T, T ? 1 : CPT, A, P, S>), grid, sh.block, lds, stream, ae, ao, \
+        hipLaunchKernelGGL((k_pair<VS, ST, W, false, T, T ? 1 : CPT, A, P, S>), grid, sh.block, lds, stream, ae, \
+                           ao, ps, partial);                                                               \
+        return hipGetLastError();                                                                          \
+    } while (0)
+    // the lagged Shan-Chen kernels (the combinations the single ticks use)
+#define BF_PM(W, T, P)                                                                                     \
+    do {                                                                                                   \
+        hipLaunchKernelGGL((k_pair<VS, ST, W, true, T, 1, false, P, kSrcOn>), grid, sh.block, 0, stream, ae, ao, \
                            ps, partial);                                                                   \
         return hipGetLastError();                                                                          \
     } while (0)
@@ -1147,6 +1158,14 @@ hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, const Pa
         else BF_P2(W, T, A, P, kSrcOff);                                                                   \
     } while (0)
     const bool closed = ae.closed != 0;
+    if (mp) {
+        if (walls) {
+            if (th) { if (closed) BF_PM(true, true, true); BF_PM(true, true, false); }
+            BF_PM(true, false, false);
+        }
+        if (th) BF_PM(false, true, false);
+        BF_PM(false, false, false);
+    }
     if (walls) {
         if (th) {
             if (closed) { if (al) BF_P(true, true, true, true); BF_P(true, true, false, true); }
@@ -1163,6 +1182,7 @@ hipError_t launch_pair_st(const StepArgs &ae_in, const StepArgs &ao_in, const Pa
     BF_P(false, false, false, false);
 #undef BF_P
 #undef BF_P2
+#undef BF_PM
 }
 
 template <int ST, bool ODD>
@@ -1222,11 +1242,11 @@ hipError_t launch_density<BF_VS>(const StepArgs &a, int storage, bool odd, bool 
 
 template <>
 hipError_t launch_pair<BF_VS>(const StepArgs &ae, const StepArgs &ao, const PairStages &ps, int storage, bool walls,
-                              bool th, const LaunchShape &sh, double *partial, hipStream_t stream) {
+                              bool mp, bool th, const LaunchShape &sh, double *partial, hipStream_t stream) {
     switch (storage) {
-    case ST_FP32: return launch_pair_st<ST_FP32>(ae, ao, ps, walls, th, sh, partial, stream);
-    case ST_FP16: return launch_pair_st<ST_FP16>(ae, ao, ps, walls, th, sh, partial, stream);
-    case ST_BF16: return launch_pair_st<ST_BF16>(ae, ao, ps, walls, th, sh, partial, stream);
+    case ST_FP32: return launch_pair_st<ST_FP32>(ae, ao, ps, walls, mp, th, sh, partial, stream);
+    case ST_FP16: return launch_pair_st<ST_FP16>(ae, ao, ps, walls, mp, th, sh, partial, stream);
+    case ST_BF16: return launch_pair_st<ST_BF16>(ae, ao, ps, walls, mp, th, sh, partial, stream);
     }
     return hipErrorInvalidValue;
 }

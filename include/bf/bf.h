@@ -115,9 +115,11 @@ typedef enum bf_param {
      * what the previous launch's even tick wrote from the last-level cache;
      * 0 = chunks of about 120 MB of lattice for every lattice that exceeds
      * 384 MB (where it was measured faster), else whole-grid ticks.  Results are
-     * identical in every mode (each cell sees the same operations); the
-     * Shan-Chen solver always runs whole-grid ticks, and so does a grid of
-     * fewer than four chunks.  No counterpart in the reference. */
+     * identical in every mode (each cell sees the same operations).  The
+     * synchronous Shan-Chen mode always runs whole-grid ticks, and so does a
+     * grid of fewer than four chunks; the lagged Shan-Chen mode pairs only
+     * with k > 0 (0 runs it in whole-grid ticks, which measured faster).  No
+     * counterpart in the reference. */
     BF_PARAM_TICK_PAIR_PLANES = 10,
     /* 1 float, 0, 2 or 4 (default 0): ticks per chunked sweep.  4 runs two
      * tick pairs in one sweep over the chunks: each launch also carries the
@@ -125,7 +127,7 @@ typedef enum bf_param {
      * has finished, so every tick of a plane reads what the previous launch
      * wrote.  A bf_step call that saves fields or sums on its last tick ends
      * with a single pair.  0 = 4 with chunks of about 80 MB for 16-bit
-     * lattices without boundary ids or a thermal lattice when
+     * lattices without boundary ids, a thermal lattice or Shan-Chen when
      * BF_PARAM_TICK_PAIR_PLANES is automatic (where it was measured faster),
      * else 2.  Results are identical in every mode.  No counterpart in the
      * reference. */
