@@ -406,6 +406,24 @@ def porous_sample(bfa, ticks, n=384):
     print(f"porous sample: {n}^3, {ticks} ticks per storage type", flush=True)
 
 
+def tgv_sample(bfa, ticks, n=256):
+    """Short workload for rocprofv3: the 256^3 TGV in FP32, FP16 and BF16,
+    each 200 ticks past the power-management transient after set-up, then
+    `ticks` ticks in one call (no saves: whole tick pairs, or sweeps of two
+    pairs for 16-bit storage, when `ticks` is a multiple of four)."""
+    import torch  # noqa: F401  (one HIP runtime with torch)
+    rho, u = taylor_green(n)
+    for st in ("fp32", "fp16", "bf16"):
+        s = bfa.Solver((n, n, n), 0.55, velocity_set="D3Q19", storage=st)
+        s.init_fields(rho, u)
+        s.step(200)
+        s.synchronize()
+        s.step(ticks)
+        s.synchronize()
+        s.close()
+    print(f"TGV sample: {n}^3, 200 + {ticks} ticks per storage type", flush=True)
+
+
 def spinodal_sample(bfa, ticks, n=192):
     """Short workload for rocprofv3: `ticks` ticks of the D3Q27 Shan-Chen +
     D3Q7 thermal workload with the lagged pseudopotential."""
@@ -469,6 +487,8 @@ def main():
                     help="run only the short workload used for rocprofv3 --pmc passes")
     ap.add_argument("--porous-sample", type=int, default=0, metavar="TICKS",
                     help="run only TICKS ticks of the 384^3 porous duct per storage type (for rocprofv3)")
+    ap.add_argument("--tgv-sample", type=int, default=0, metavar="TICKS",
+                    help="run only 200 + TICKS ticks of the 256^3 TGV per storage type (for rocprofv3)")
     ap.add_argument("--spinodal-sample", type=int, default=0, metavar="TICKS",
                     help="run only TICKS ticks of the 192^3 Shan-Chen + thermal workload (for rocprofv3)")
     ap.add_argument("--transient-sample", action="store_true",
@@ -487,6 +507,9 @@ def main():
         return
     if args.porous_sample:
         porous_sample(bfa, args.porous_sample)
+        return
+    if args.tgv_sample:
+        tgv_sample(bfa, args.tgv_sample)
         return
     if args.spinodal_sample:
         spinodal_sample(bfa, args.spinodal_sample)
