@@ -375,9 +375,10 @@ def cpu_leg(bfa, storages, n_err=64, ticks_err=1000, n=128, ticks=400):
     return base, errors
 
 
-def counters_sample(bfa, n=256, ticks=10):
+def counters_sample(bfa, n=256, ticks=12):
     """Short workload for one rocprofv3 --pmc pass: a 1 GiB device copy of known
-    traffic (counter calibration), then `ticks` ticks of the FP32 and FP16 runs."""
+    traffic (counter calibration), then `ticks` ticks of the FP32 and FP16 runs
+    (12: six tick pairs in FP32, three sweeps of two pairs in FP16)."""
     import torch  # noqa: F401  (one HIP runtime with torch)
     print("copy GB/s", round(probe.copy_bandwidth(1 << 30, 3), 1), flush=True)
     rho, u = taylor_green(n)
