@@ -11,7 +11,8 @@ Prints one JSON line:
   fp16_shifted, bf16_...  the same run with 16-bit shifted storage
   error_vs_fp32           16-bit fields vs the FP32 run (same workload at 128^3)
   error_vs_fp64           every storage type vs the FP64 CPU oracle (TGV and porous duct
-                          at 64^3, 1000 ticks; computed in the CPU leg)
+                          at 64^3, 1000 ticks; Shan-Chen + thermal FP32 at 32^3, 500
+                          ticks; computed in the CPU leg)
   other_configs           porous 384^3 D3Q19 (FP32/FP16/BF16, + error at 128^3),
                           Shan-Chen D3Q27 + thermal D3Q7 192^3, D2Q9 cavity 256^2
   adjoint_over_forward    differentiable channel: gradient time / forward time
@@ -371,6 +372,33 @@ def cpu_leg(bfa, storages, n_err=64, ticks_err=1000, n=128, ticks=400):
     for st in storages:
         vd = _gpu_fields(bfa, (m, m, m), 0.56, st, ticks_err, setup, periodic=(1, 0, 0))[fluid]
         errors["porous"][st] = field_errors(vd, ref)
+    # D3Q27 Shan-Chen (lagged psi, G -5, tau 1) + D3Q7 thermal hot/cold walls,
+    # FP32 (the configuration's storage), at 32^3 for 500 ticks
+    ms, ticks_sc = 32, 500
+    rho_s, u_s, ids_s = spinodal(ms, ms, ms)
+    cs = ms ** 3
+    fluid_s = ids_s == 0
+
+    def setup_sc(s):
+        s.set_object(1, temperature=1.0)
+        s.set_object(2, temperature=0.0)
+        s.set_param("interaction_strength", -5.0)
+        s.set_param("reference_temperature", 0.5)
+        s.init_fields(rho_s, u_s, force=np.tile(np.array([0, -1e-5, 0], np.float32), (cs, 1)),
+                      temperature=np.full(cs, 0.5, np.float32), boundary_id=ids_s)
+
+    o = oracle.Oracle((ms,) * 3, 1.0, periodic=(1, 0, 1), velocity_set="D3Q27", multiphase=True, precision="f64")
+    setup_sc(o)
+    o.step(ticks_sc, save=True)
+    ref = o.read("velocity_density")[fluid_s]
+    o.close()
+    g = bfa.Solver((ms,) * 3, 1.0, periodic=(1, 0, 1), velocity_set="D3Q27", multiphase=True)
+    setup_sc(g)
+    g.step(ticks_sc, save=True)
+    vd = g.read("velocity_density").astype(np.float64)[fluid_s]
+    g.close()
+    errors["shan_chen_thermal"] = {"workload": f"D3Q27 Shan-Chen + D3Q7 thermal walls {ms}^3, {ticks_sc} ticks",
+                                   "fp32": field_errors(vd, ref)}
     errors["oracle_seconds"] = round(time.perf_counter() - t0, 1)
     return base, errors
 
