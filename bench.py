@@ -12,7 +12,8 @@ Prints one JSON line:
   error_vs_fp32           16-bit fields vs the FP32 run (same workload at 128^3)
   error_vs_fp64           every storage type vs the FP64 CPU oracle (TGV and porous duct
                           at 64^3, 1000 ticks; Shan-Chen + thermal FP32 at 32^3, 500
-                          ticks; computed in the CPU leg)
+                          ticks; the D2Q9 channel gradient against the FP64 adjoint;
+                          computed in the CPU leg)
   other_configs           porous 384^3 D3Q19 (FP32/FP16/BF16, + error at 128^3),
                           Shan-Chen D3Q27 + thermal D3Q7 192^3, D2Q9 cavity 256^2
   adjoint_over_forward    differentiable channel: gradient time / forward time
@@ -399,6 +400,24 @@ def cpu_leg(bfa, storages, n_err=64, ticks_err=1000, n=128, ticks=400):
     g.close()
     errors["shan_chen_thermal"] = {"workload": f"D3Q27 Shan-Chen + D3Q7 thermal walls {ms}^3, {ticks_sc} ticks",
                                    "fp32": field_errors(vd, ref)}
+    # the differentiable D2Q9 channel (512 x 128, 2000 ticks, checkpoints every
+    # 100): the GPU gradient against the FP64 CPU adjoint of the same schedule
+    res_c, ticks_c = (512, 128, 1), 2000
+    gam = porosity(*res_c)
+    oc = oracle.OracleChannel(res_c, 0.6, 0.05, velocity_set="D2Q9", precision="f64")
+    oc.set_porosity(gam)
+    J64, dg64, du64 = oc.gradient_checkpointed(ticks_c, 100)
+    ch = bfa.capi.Channel(res_c, 0.6, 0.05, velocity_set="D2Q9")
+    ch.set_porosity(gam)
+    Jg, dgg, dug = ch.gradient(ticks_c, checkpoint_every=100, trajectory_budget_bytes=0)
+    ch.close()
+    rows = np.r_[0, np.ones(res_c[1] - 2), 0].repeat(res_c[0]) > 0
+    errors["channel_gradient"] = {
+        "workload": f"channel D2Q9 {res_c[0]}x{res_c[1]}, {ticks_c} ticks, checkpoint every 100, FP32 on the GPU "
+                    f"against the FP64 adjoint",
+        "J": abs(Jg - J64) / abs(J64),
+        "dJ_dgamma": float(np.linalg.norm(dgg[rows] - dg64[rows]) / np.linalg.norm(dg64[rows])),
+        "dJ_duin": abs(dug - du64) / abs(du64)}
     errors["oracle_seconds"] = round(time.perf_counter() - t0, 1)
     return base, errors
 
